@@ -1,0 +1,238 @@
+/*
+ * mps.h -- C ABI of the MI355X (gfx950) explicit-MPS time step.
+ *
+ * The reference has no FFI: its step sits behind the all-public C++ class
+ * `MPS` (class.cuh:8-28), driven by main.cu:16-24 and MPS::ClcMPS
+ * (kernel.cu:1872-1945).  Every entry point below replaces one of those
+ * methods or the state they expose through public `d_*` pointers; the
+ * replaced reference symbol is cited on each declaration.
+ *
+ * Conventions
+ *  - Every function returns an `int` status (MPS_OK or an MPS_E_* code) and
+ *    never exits the process (the reference's CHECK macro calls exit(1),
+ *    const.cuh:87-95).  mps_last_error() returns the calling thread's last
+ *    message.
+ *  - Host arrays are owned by the caller and passed as pointer + count.
+ *    Vector fields use the reference's SoA layout (areal3, const.cuh:67-71):
+ *    one contiguous block per component, i.e. x[0..n), y[0..n), z[0..n).
+ *    The z block is present (and ignored) for dim = 2.
+ *  - Host-side values are double; an FP32 handle rounds them on upload.
+ *  - All device work of a handle is issued on one HIP stream (settable);
+ *    mps_step issues no host synchronisation.
+ */
+#ifndef MPS_H
+#define MPS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* particle type codes, const.cuh:28-33 */
+#define MPS_GST (-1)
+#define MPS_FLD 0
+#define MPS_WLL 1
+#define MPS_OBJ 2
+#define MPS_OBJ2 3
+#define MPS_MRR 4
+
+/* wall classification, const.cuh:35-36 */
+#define MPS_SURFACE 0
+#define MPS_EDGE 1
+
+/* mirror slots per fluid particle, const.cuh:50 */
+#define MPS_NUM_MRR 10
+
+enum mps_status {
+  MPS_OK = 0,
+  MPS_E_ARG = 1,   /* invalid argument or scene */
+  MPS_E_HIP = 2,   /* HIP runtime error (no device, launch failure, ...) */
+  MPS_E_OOM = 3,   /* device allocation failed */
+  MPS_E_STATE = 4, /* call not valid in the handle's current state */
+};
+
+enum mps_precision { MPS_FP32 = 0, MPS_FP64 = 1 };
+
+/* Pressure upper clamp of d_MkPrs (kernel.cu:1392-1406).  The reference sets
+ * Pmax = 1000*G.y*H (kernel.cu:411), which is negative for G.y < 0. */
+enum mps_pmax_mode {
+  MPS_PMAX_NONE = 0,      /* clip at 0 only (default) */
+  MPS_PMAX_ABS_G = 1,     /* Pmax = rho*|G.y|*H */
+  MPS_PMAX_REFERENCE = 2, /* Pmax = rho*G.y*H, the reference's literal value */
+};
+
+/* Physical and numerical parameters: the 17 values of MPS_physical.txt
+ * (kernel.cu:12-22) plus the constants the reference hard-codes
+ * (const.cuh:38-46, kernel.cu:309-315, 1184).  Fill with mps_params_default
+ * and then override. */
+typedef struct mps_params {
+  int dim;            /* 2 or 3 (the reference is 3-D only, kernel.cu:341-343) */
+  int precision;      /* enum mps_precision; `real` of const.cuh:52-53 */
+  int pmax_mode;      /* enum mps_pmax_mode */
+  int reserved0;
+  double l0;          /* PCL_DST, particle spacing */
+  double range_min[3];/* MPS_physical.txt line 1 (before the re expansion, kernel.cu:26-31) */
+  double range_max[3];/* MPS_physical.txt line 2 */
+  double gravity[3];  /* G */
+  double rho;         /* Dns_FLD = Dns_WLL = Dns_OBJ = Dns_OBJ2 (const.cuh:38-41) */
+  double nu;          /* KNM_VSC, kinematic viscosity */
+  double c;           /* sound speed SND; <= 0 derives umax/Ma (kernel.cu:408) */
+  double mach;        /* Ma, used only when c <= 0 */
+  double dt;          /* time step; <= 0 derives l0/(umax+SND) (kernel.cu:415) */
+  double restitution; /* COL_RAT; COL = 1 + COL_RAT (kernel.cu:373) */
+  double dst_lmt_rat; /* DST_LMT_RAT; rlim = l0*DST_LMT_RAT (kernel.cu:370) */
+  double re_factor;   /* 3.1: r, bucket width, domain expansion, ChkPcl inset */
+  double rp_factor;   /* 2.1: pressure / gradient radius (kernel.cu:311) */
+  double rlim_wall_factor; /* 0.45: wall contact radius of d_ChkCol (kernel.cu:1184) */
+  double edge_factor; /* radius / l0 of the one-time wall edge test (kernel.cu:558 uses 1.0) */
+} mps_params;
+
+/* Initial particle set, the content of initial.txt after RdDat
+ * (kernel.cu:110-131).  `normal` is the inward wall normal WLLVec (RdDat
+ * negates the file's normals, kernel.cu:127).  Optional arrays may be NULL:
+ * vel/prs/pav default to 0, edge is computed (d_Surface_Edge,
+ * kernel.cu:553-668) with radius edge_factor*l0. */
+typedef struct mps_scene {
+  int n;
+  int reserved0;
+  const int8_t* typ;    /* [n] */
+  const double* pos;    /* [3][n] */
+  const double* vel;    /* [3][n] or NULL */
+  const double* normal; /* [3][n] (read for wall types only) */
+  const double* prs;    /* [n] or NULL */
+  const double* pav;    /* [n] or NULL */
+  const int8_t* edge;   /* [n] WLLSE or NULL */
+} mps_scene;
+
+/* Derived constants, computed by SetPara (kernel.cu:334-417) and AlcBkt
+ * (kernel.cu:307-331).  Reported in double whatever the precision. */
+typedef struct mps_constants {
+  double n0, lmd, n0_grad, lmd_grad;
+  int num_n0_grad;
+  int nb[3];          /* nBx, nBy, nBz (nBz = 1 for dim 2) */
+  int ncell;          /* nBxyz */
+  int n_fluid;        /* particles that are not walls (FLD + GST at creation) */
+  int n_wall;         /* WLL/OBJ/OBJ2 particles */
+  int reserved0;
+  double r, rp, rlim, rlim_wall, db, dbinv;
+  double min_c[3], max_c[3];
+  double umax, snd, prs_coef, pmax, dt, col, max_height, min_height;
+} mps_constants;
+
+typedef struct mps_handle mps_handle;
+
+/* Step phases, in the order of MPS::ClcMPS (kernel.cu:1904-1935). */
+enum mps_phase {
+  MPS_PH_MKBKT = 0,  /* MkBkt, kernel.cu:532-550 */
+  MPS_PH_MIRROR = 1, /* ResetMRR + GenMRR_nonslip + MkBkt_MRR, kernel.cu:704-1010 */
+  MPS_PH_VSC = 2,    /* VscTrm, kernel.cu:1122-1135 */
+  MPS_PH_UP1 = 3,    /* UpPcl1, kernel.cu:1152-1165 */
+  MPS_PH_COL = 4,    /* ChkCol (+ d_UpChkCol), kernel.cu:1289-1303 */
+  MPS_PH_PRS = 5,    /* MkPrs, kernel.cu:1414-1427 */
+  MPS_PH_GRAD = 6,   /* PrsGrdTrm, kernel.cu:1798-1812 */
+  MPS_PH_UP2 = 7,    /* UpPcl2, kernel.cu:1831-1844 */
+};
+
+/* which bucket set mps_get_buckets returns */
+enum mps_bucket_set { MPS_BKT_FLUID = 0, MPS_BKT_WALL = 1, MPS_BKT_MIRROR = 2 };
+
+/* ---- scene generation (host only, no device needed) ------------------- */
+
+/* Dam-break scene: an open-top tank of interior `tank` with `wall_layers`
+ * wall layers on the sides and bottom, and a water column `column` in the
+ * x-min / y-min / z-min corner, all on a square lattice of spacing l0.
+ * Fluid particles come first (ids 0..n_fluid-1), walls after.  `jitter`
+ * (fraction of l0) displaces fluid particles by a seeded uniform offset.
+ * Replaces the author's initial.txt / MPS_physical.txt (not shipped). */
+typedef struct mps_dambreak_spec {
+  int dim;
+  int wall_layers;
+  uint64_t seed;
+  double l0;
+  double tank[3];
+  double column[3];
+  double jitter;     /* uniform in +-jitter*l0 */
+  double domain_top; /* range_max.y */
+} mps_dambreak_spec;
+
+/* mps_scene_dambreak_count: number of fluid and wall particles. */
+int mps_scene_dambreak_count(const mps_dambreak_spec* spec, int* n_fluid, int* n_wall);
+/* Fills typ[n], pos[3][n], normal[3][n] (n = n_fluid + n_wall) and the
+ * range_min/range_max pair to put into mps_params. */
+int mps_scene_dambreak_fill(const mps_dambreak_spec* spec, int8_t* typ, double* pos,
+                            double* normal, double* range_min, double* range_max);
+
+/* ---- handle lifecycle ----------------------------------------------- */
+
+/* Reference defaults for the hard-coded constants (3.1, 2.1, 0.45, 1000). */
+int mps_params_default(mps_params* p);
+
+/* RdDat + AlcBkt + SetPara + Surface_Edge (main.cu:16-22, kernel.cu:1878-1879):
+ * validates the scene, computes the constants, uploads the state. */
+int mps_create(const mps_params* params, const mps_scene* scene, mps_handle** out);
+int mps_destroy(mps_handle* h);
+
+/* HIP stream (hipStream_t) for all later work; NULL = default stream. */
+int mps_set_stream(mps_handle* h, void* stream);
+int mps_sync(mps_handle* h);
+
+/* ---- the hot path ----------------------------------------------------- */
+
+/* nsteps iterations of the ClcMPS loop body (kernel.cu:1904-1941) with
+ * fused kernels.  Asynchronous on the handle's stream. */
+int mps_step(mps_handle* h, int nsteps);
+
+/* One phase (enum mps_phase) with unfused kernels, for per-phase checks. */
+int mps_phase(mps_handle* h, int phase);
+
+/* ---- state access ----------------------------------------------------- */
+
+int mps_get_constants(mps_handle* h, mps_constants* out);
+
+/* DevicetoHost (kernel.cu:1847-1869): state in creation order.  Any pointer
+ * may be NULL.  Vector fields are [3][n].  acc is the reference's Acc array
+ * (zero after a full step).  nd is the last number density of d_MkPrs. */
+int mps_get_state(mps_handle* h, int8_t* typ, double* pos, double* vel, double* acc,
+                  double* prs, double* pav, double* nd);
+
+/* Overwrites the state (same layout as mps_get_state; NULL keeps a field).
+ * Wall positions are static and may not change. */
+int mps_set_state(mps_handle* h, const int8_t* typ, const double* pos, const double* vel,
+                  const double* prs, const double* pav);
+
+/* Bucket arrays of the last build: cell_start[ncell+1] and the particle ids
+ * (creation index; mirror id = parent*MPS_NUM_MRR + slot) in cell order and
+ * ascending id inside a cell.  *count receives the number of ids. ids may be
+ * NULL to query the count.  Replaces bfst/nxt (kernel.cu:516-529, 916-929). */
+int mps_get_buckets(mps_handle* h, int which, int* cell_start, int* ids, int* count);
+
+/* Live mirror particles of the last mirror phase in bucket order: ids,
+ * pos[3][m], prs[m] (PosM/PrsM/TypM, class.cuh:67-76).  VelM is always 0. */
+int mps_get_mirrors(mps_handle* h, int* ids, double* pos, double* prs, int* count);
+
+const char* mps_last_error(void);
+
+/* ---- host-only helpers (no device needed) ----------------------------- */
+
+/* SetPara + AlcBkt constants for a scene without creating a handle
+ * (kernel.cu:307-417). */
+int mps_compute_constants(const mps_params* params, const mps_scene* scene, mps_constants* out);
+
+/* Surface_Edge once on the host (kernel.cu:553-686): edge[n] = WLLSE. */
+int mps_compute_edge(const mps_params* params, const mps_scene* scene, int8_t* edge);
+
+/* ---- diagnostics --------------------------------------------------------- */
+
+/* Runs the device's sqrt(a) and a/b on n values (precision MPS_FP32 rounds
+ * the inputs to float first); the results are returned as double.  Used to
+ * show that device and host arithmetic agree bit for bit.  No reference
+ * counterpart. */
+int mps_probe_arith(int precision, const double* a, const double* b, double* sqrt_out,
+                    double* div_out, int n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MPS_H */

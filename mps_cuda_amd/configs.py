@@ -1,0 +1,61 @@
+"""The dam-break workloads (scenes + parameters) used by tests and bench.
+
+  cfg0  2-D dam break, FP64: tank 1.0 x 0.6, l0 = 0.01, column 0.25 x 0.5
+        (1,250 fluid), 3 wall layers, dt = 5e-4, 2000 steps.
+  cfg1  3-D dam break, FP32: tank 1.0 x 0.6 x 0.3, l0 = 0.005, column
+        0.25 x 0.5 x 0.3 (300,000 fluid), jitter +-0.01 l0 (seed 1),
+        dt = 2.5e-4, 2000 steps.
+  cfg2a 3-D dam break at l0 = 0.0025 (2.4 M fluid), 500 steps.
+Common constants: rho = 1000, nu = 1e-6, g = -9.81, c = 22 m/s, collision
+distance 0.5 l0, restitution 0.2, re = 3.1 l0, rp = 2.1 l0.
+
+Pressure clamp: pmax_mode = PMAX_ABS_G, the reference's upper clamp
+(kernel.cu:411, 1392-1406) with |G.y| in place of G.y.  With no upper clamp
+the stated time steps give an acoustic Courant number c*dt/l0 = 1.1 and the
+explicit pressure diverges within 200 steps.
+"""
+from __future__ import annotations
+
+from . import ffi
+
+COMMON = dict(rho=1000.0, nu=1e-6, gravity=(0.0, -9.81, 0.0), c=22.0, restitution=0.2,
+              dst_lmt_rat=0.5, re_factor=3.1, rp_factor=2.1, pmax_mode=ffi.PMAX_ABS_G)
+
+
+def params_for(scene: ffi.SceneArrays, dim: int, l0: float, dt: float, precision, **kw):
+    args = dict(COMMON)
+    args.update(kw)
+    return ffi.default_params(dim=dim, precision=precision, l0=l0, dt=dt,
+                              range_min=scene.range_min, range_max=scene.range_max, **args)
+
+
+def cfg0(precision="fp64", **kw):
+    sc = ffi.dambreak(dim=2, l0=0.01, tank=(1.0, 0.6, 0.0), column=(0.25, 0.5, 0.0),
+                      wall_layers=3, jitter=0.0, seed=0, domain_top=1.0)
+    return sc, params_for(sc, 2, 0.01, 5e-4, precision, **kw)
+
+
+def cfg1(precision="fp32", **kw):
+    sc = ffi.dambreak(dim=3, l0=0.005, tank=(1.0, 0.6, 0.3), column=(0.25, 0.5, 0.3),
+                      wall_layers=3, jitter=0.01, seed=1, domain_top=1.0)
+    return sc, params_for(sc, 3, 0.005, 2.5e-4, precision, **kw)
+
+
+def cfg2a(precision="fp32", **kw):
+    sc = ffi.dambreak(dim=3, l0=0.0025, tank=(1.0, 0.6, 0.3), column=(0.25, 0.5, 0.3),
+                      wall_layers=3, jitter=0.01, seed=2, domain_top=1.0)
+    return sc, params_for(sc, 3, 0.0025, 1.25e-4, precision, **kw)
+
+
+def small3d(precision="fp64", l0=0.02, jitter=0.01, seed=5, **kw):
+    """A small 3-D dam break (1,000 fluid) for exhaustive parity checks."""
+    sc = ffi.dambreak(dim=3, l0=l0, tank=(0.4, 0.3, 0.2), column=(0.1, 0.2, 0.1),
+                      wall_layers=3, jitter=jitter, seed=seed, domain_top=0.5)
+    return sc, params_for(sc, 3, l0, 1e-3 * l0 / 0.01 * 0.5, precision, **kw)
+
+
+def small2d(precision="fp64", l0=0.01, jitter=0.0, seed=0, **kw):
+    """A small 2-D dam break (200 fluid, 10 x 20)."""
+    sc = ffi.dambreak(dim=2, l0=l0, tank=(0.3, 0.3, 0.0), column=(0.1, 0.2, 0.0),
+                      wall_layers=3, jitter=jitter, seed=seed, domain_top=0.5)
+    return sc, params_for(sc, 2, l0, 5e-4, precision, **kw)

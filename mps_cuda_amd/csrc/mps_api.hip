@@ -1,0 +1,762 @@
+// mps_api.hip -- C ABI (include/mps.h) over the gfx950 kernels.
+//
+// One handle = one scene on one GPU.  The handle owns every device buffer;
+// all work runs on one HIP stream with no host synchronisation inside
+// mps_step (the reference synchronises after every phase, 18 times a step:
+// kernel.cu:544-1841).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/mps.h"
+#include "mps_host.h"
+#include "mps_kernels.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int set_err(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                    \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return set_err(e_ == hipErrorOutOfMemory ? MPS_E_OOM : MPS_E_HIP,                  \
+                     std::string(#expr) + ": " + hipGetErrorString(e_));                 \
+  } while (0)
+
+inline int nblk(long long n, int cap = 1 << 30) {
+  long long b = (n + mps::kBlock - 1) / mps::kBlock;
+  if (b < 1) b = 1;
+  return (int)std::min<long long>(b, cap);
+}
+
+struct ImplBase {
+  virtual ~ImplBase() {}
+  virtual int init(const mps_params& p, const mps_scene& s) = 0;
+  virtual int step(int n) = 0;
+  virtual int phase(int ph) = 0;
+  virtual int get_state(int8_t*, double*, double*, double*, double*, double*, double*) = 0;
+  virtual int set_state(const int8_t*, const double*, const double*, const double*,
+                        const double*) = 0;
+  virtual int buckets(int which, int* cs, int* ids, int* count) = 0;
+  virtual int mirrors(int* ids, double* pos, double* prs, int* count) = 0;
+  virtual void constants(mps_constants* out) = 0;
+  hipStream_t stream = nullptr;
+  std::vector<void*> allocs;
+  int alloc_raw(void** p, size_t bytes) {
+    if (bytes == 0) bytes = 256;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess)
+      return set_err(e == hipErrorOutOfMemory ? MPS_E_OOM : MPS_E_HIP,
+                     std::string("hipMalloc: ") + hipGetErrorString(e));
+    allocs.push_back(*p);
+    return MPS_OK;
+  }
+  template <class T>
+  int alloc(T** p, size_t count) {
+    return alloc_raw((void**)p, count * sizeof(T));
+  }
+  void free_all() {
+    for (void* p : allocs) (void)hipFree(p);
+    allocs.clear();
+  }
+};
+
+template <class real, int DIM>
+struct Impl : ImplBase {
+  using V = mps::Vec<real, DIM>;
+  mps_params prm;
+  mps::HostConst<real> hk;
+  mps::Cst<real> c;
+  int n_total = 0, nf = 0, nw = 0, mcap = 0, ncell = 0;
+  bool acc_live = false, wall_pav = false;
+  // creation index of each fluid-set / wall-set entry at creation
+  std::vector<int> host_fluid_ids, host_wall_ids;
+
+  // fluid set, [0] = current, [1] = scratch
+  V *fpos[2], *fvel[2], *facc[2];
+  real *fprs[2], *fpav[2], *fnd[2];
+  int8_t* ftyp[2];
+  int* fid[2];
+  int* fcs = nullptr;  // ncell + 2
+  // walls (static)
+  V *wpos = nullptr, *wvel = nullptr, *wnrm = nullptr;
+  int8_t* wedge = nullptr;
+  int* wcs = nullptr;
+  real *wprs = nullptr, *wpav = nullptr;
+  // mirrors
+  V *mraw_pos = nullptr, *mpos = nullptr;
+  real *mraw_prs = nullptr, *mprs = nullptr;
+  int *mraw_id = nullptr, *mid = nullptr, *mcs = nullptr, *mcount = nullptr;
+  // bucket scratch (shared by the fluid and mirror builds)
+  int *key = nullptr, *slot = nullptr, *tmp = nullptr, *perm = nullptr, *cnt = nullptr;
+  // scan levels: parts[l] holds block totals, parts_ex[l] their exclusive scan
+  std::vector<int*> parts, parts_ex;
+  std::vector<int> part_n;
+
+  ~Impl() override { free_all(); }
+
+  void fill_cst() {
+    for (int a = 0; a < 3; a++) {
+      c.minc[a] = hk.minc[a];
+      c.maxc[a] = hk.maxc[a];
+      c.lo[a] = hk.minc[a] + hk.inset;  // kernel.cu:484-486
+      c.hi[a] = hk.maxc[a] - hk.inset;
+      c.g[a] = hk.g[a];
+      c.nb[a] = hk.nb[a];
+    }
+    c.dbinv = hk.dbinv;
+    c.l0 = hk.l0;
+    c.r = hk.r;
+    c.r2 = hk.r2;
+    c.rp = hk.rp;
+    c.rp2 = hk.rp2;
+    c.rlim2 = hk.rlim2;
+    c.rlim_wall2 = hk.rlim_wall2;
+    c.col = hk.col;
+    c.mass = hk.mass;
+    c.n0knm = (real(1) / hk.n0) * hk.nu;   // kernel.cu:1029, 1113
+    c.grad_a = real(DIM) / hk.n0_grad;     // kernel.cu:1665 (A3 = 3/n0_grad in 3-D)
+    c.invro = real(-1) / hk.mass;          // kernel.cu:1659
+    c.dt = hk.dt;
+    c.umax = hk.umax;
+    c.prs_coef = hk.prs_coef;
+    c.n0_grad = hk.n0_grad;
+    c.pmax = hk.pmax;
+    c.nbxy = hk.nbxy;
+    c.ncell = hk.ncell;
+  }
+
+  int host_key(const double* pos, int n, int i) const {
+    V p = mps::mkvec<real, DIM>(real(pos[i]), real(pos[(size_t)n + i]),
+                                DIM == 3 ? real(pos[2 * (size_t)n + i]) : real(0));
+    mps::Cell<real, DIM> q = mps::cell_of<real, DIM>(p, c);
+    return mps::cell_key(q.ix, q.iy, q.iz, c);
+  }
+
+  int init(const mps_params& p, const mps_scene& s) override {
+    prm = p;
+    std::string err;
+    int rc = mps::compute_constants<real>(p, s, hk, err);
+    if (rc != MPS_OK) return set_err(rc, err);
+    fill_cst();
+    n_total = s.n;
+    ncell = hk.ncell;
+    std::vector<int8_t> edge;
+    if (s.edge) edge.assign(s.edge, s.edge + s.n);
+    else mps::compute_edge<real>(p, s, edge);
+    for (int i = 0; i < s.n; i++) (mps::is_wall_type(s.typ[i]) ? host_wall_ids : host_fluid_ids).push_back(i);
+    nf = (int)host_fluid_ids.size();
+    nw = (int)host_wall_ids.size();
+    if ((long long)nf * mps::kNumMRR > (1ll << 30))
+      return set_err(MPS_E_ARG, "too many fluid particles for 32-bit mirror ids");
+    // walls sorted by (cell, id), keys computed in `real` exactly as on the device
+    std::vector<int> wkey(s.n, 0);
+    for (int i : host_wall_ids) wkey[i] = host_key(s.pos, s.n, i);
+    std::stable_sort(host_wall_ids.begin(), host_wall_ids.end(),
+                     [&](int a, int b) { return wkey[a] < wkey[b]; });
+    wall_typ.resize(nw);
+    for (int w = 0; w < nw; w++) wall_typ[w] = s.typ[host_wall_ids[w]];
+    mcap = std::max(1, nf * mps::kNumMRR);
+
+    for (int b = 0; b < 2; b++) {
+      if ((rc = alloc(&fpos[b], nf))) return rc;
+      if ((rc = alloc(&fvel[b], nf))) return rc;
+      if ((rc = alloc(&facc[b], nf))) return rc;
+      if ((rc = alloc(&fprs[b], nf))) return rc;
+      if ((rc = alloc(&fpav[b], nf))) return rc;
+      if ((rc = alloc(&fnd[b], nf))) return rc;
+      if ((rc = alloc(&ftyp[b], nf))) return rc;
+      if ((rc = alloc(&fid[b], nf))) return rc;
+    }
+    if ((rc = alloc(&fcs, (size_t)ncell + 2))) return rc;
+    if ((rc = alloc(&wpos, nw)) || (rc = alloc(&wvel, nw)) || (rc = alloc(&wnrm, nw)) ||
+        (rc = alloc(&wedge, nw)) || (rc = alloc(&wprs, nw)) || (rc = alloc(&wpav, nw)) ||
+        (rc = alloc(&wcs, (size_t)ncell + 1)))
+      return rc;
+    if ((rc = alloc(&mraw_pos, mcap)) || (rc = alloc(&mpos, mcap)) || (rc = alloc(&mraw_prs, mcap)) ||
+        (rc = alloc(&mprs, mcap)) || (rc = alloc(&mraw_id, mcap)) || (rc = alloc(&mid, mcap)) ||
+        (rc = alloc(&mcs, (size_t)ncell + 2)) || (rc = alloc(&mcount, 1)))
+      return rc;
+    const size_t scr = std::max(mcap, nf);
+    if ((rc = alloc(&key, scr)) || (rc = alloc(&slot, scr)) || (rc = alloc(&tmp, scr)) ||
+        (rc = alloc(&perm, scr)) || (rc = alloc(&cnt, (size_t)ncell + 1)))
+      return rc;
+    // scan levels for n = ncell + 1
+    for (long long n = ncell + 1;;) {
+      const long long nb = (n + 1 + mps::kScanTile - 1) / mps::kScanTile;
+      if (nb <= 1) break;
+      int *a, *b;
+      if ((rc = alloc(&a, nb)) || (rc = alloc(&b, nb + 1))) return rc;
+      parts.push_back(a);
+      parts_ex.push_back(b);
+      part_n.push_back((int)nb);
+      n = nb;
+    }
+
+    // upload walls
+    {
+      std::vector<V> hp(nw), hv(nw), hn(nw);
+      std::vector<int8_t> he(nw);
+      std::vector<real> hprs(nw), hpav(nw);
+      std::vector<int> hcs(ncell + 1, 0);
+      const size_t n = s.n;
+      for (int w = 0; w < nw; w++) {
+        const int i = host_wall_ids[w];
+        auto get = [&](const double* a, int k) { return a ? real(a[k * n + i]) : real(0); };
+        hp[w] = mps::mkvec<real, DIM>(get(s.pos, 0), get(s.pos, 1), DIM == 3 ? get(s.pos, 2) : real(0));
+        hv[w] = mps::mkvec<real, DIM>(get(s.vel, 0), get(s.vel, 1), DIM == 3 ? get(s.vel, 2) : real(0));
+        hn[w] = mps::mkvec<real, DIM>(get(s.normal, 0), get(s.normal, 1),
+                                      DIM == 3 ? get(s.normal, 2) : real(0));
+        he[w] = edge[i];
+        hprs[w] = s.prs ? real(s.prs[i]) : real(0);
+        hpav[w] = s.pav ? real(s.pav[i]) : real(0);
+        wall_pav = wall_pav || hprs[w] != real(0);
+        hcs[wkey[i] + 1]++;
+      }
+      for (int k = 0; k < ncell; k++) hcs[k + 1] += hcs[k];
+      HIP_TRY(hipMemcpy(wpos, hp.data(), nw * sizeof(V), hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(wvel, hv.data(), nw * sizeof(V), hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(wnrm, hn.data(), nw * sizeof(V), hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(wedge, he.data(), nw, hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(wprs, hprs.data(), nw * sizeof(real), hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(wpav, hpav.data(), nw * sizeof(real), hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(wcs, hcs.data(), (ncell + 1) * sizeof(int), hipMemcpyHostToDevice));
+    }
+    // upload the fluid set in creation order, then bucket it (ClcMPS starts
+    // with MkBkt, kernel.cu:1878)
+    {
+      std::vector<V> hp(nf), hv(nf), ha(nf);
+      std::vector<real> hprs(nf), hpav(nf), hnd(nf, real(0));
+      std::vector<int8_t> ht(nf);
+      const size_t n = s.n;
+      for (int f = 0; f < nf; f++) {
+        const int i = host_fluid_ids[f];
+        auto get = [&](const double* a, int k) { return a ? real(a[k * n + i]) : real(0); };
+        const bool dead = s.typ[i] == MPS_GST;
+        hp[f] = mps::mkvec<real, DIM>(get(s.pos, 0), get(s.pos, 1), DIM == 3 ? get(s.pos, 2) : real(0));
+        hv[f] = mps::mkvec<real, DIM>(get(s.vel, 0), get(s.vel, 1), DIM == 3 ? get(s.vel, 2) : real(0));
+        ha[f] = mps::mkvec<real, DIM>(0, 0, 0);
+        hprs[f] = s.prs ? real(s.prs[i]) : real(0);
+        hpav[f] = s.pav ? real(s.pav[i]) : real(0);
+        ht[f] = dead ? MPS_GST : MPS_FLD;
+      }
+      HIP_TRY(hipMemcpy(fpos[0], hp.data(), nf * sizeof(V), hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(fvel[0], hv.data(), nf * sizeof(V), hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(facc[0], ha.data(), nf * sizeof(V), hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(fprs[0], hprs.data(), nf * sizeof(real), hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(fpav[0], hpav.data(), nf * sizeof(real), hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(fnd[0], hnd.data(), nf * sizeof(real), hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(ftyp[0], ht.data(), nf, hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(fid[0], host_fluid_ids.data(), nf * sizeof(int), hipMemcpyHostToDevice));
+    }
+    hipLaunchKernelGGL(mps::k_fill_int, dim3(nblk(ncell + 2, 4096)), dim3(mps::kBlock), 0, stream,
+                       mcs, ncell + 2, 0);
+    hipLaunchKernelGGL(mps::k_fill_int, dim3(1), dim3(mps::kBlock), 0, stream, mcount, 1, 0);
+    if ((rc = bucket())) return rc;
+    HIP_TRY(hipStreamSynchronize(stream));
+    return MPS_OK;
+  }
+
+  // exclusive scan of in[0..n) into out[0..n] (out[n] = total)
+  void scan(const int* in, int n, int* out, int level) {
+    const int nb = (n + 1 + mps::kScanTile - 1) / mps::kScanTile;
+    if (nb <= 1) {
+      hipLaunchKernelGGL(mps::k_scan_down, dim3(1), dim3(mps::kBlock), 0, stream, in, n,
+                         (const int*)nullptr, out);
+      return;
+    }
+    hipLaunchKernelGGL(mps::k_scan_reduce, dim3(nb), dim3(mps::kBlock), 0, stream, in, n, parts[level]);
+    scan(parts[level], nb, parts_ex[level], level + 1);
+    hipLaunchKernelGGL(mps::k_scan_down, dim3(nb), dim3(mps::kBlock), 0, stream, in, n,
+                       (const int*)parts_ex[level], out);
+  }
+
+  mps::FluidArrays<real, DIM> fa(int b) {
+    return {fpos[b], fvel[b], acc_live ? facc[b] : nullptr, fprs[b], fpav[b], fnd[b], ftyp[b], fid[b]};
+  }
+  void swap_all() {
+    std::swap(fpos[0], fpos[1]);
+    std::swap(fvel[0], fvel[1]);
+    std::swap(facc[0], facc[1]);
+    std::swap(fprs[0], fprs[1]);
+    std::swap(fpav[0], fpav[1]);
+    std::swap(fnd[0], fnd[1]);
+    std::swap(ftyp[0], ftyp[1]);
+    std::swap(fid[0], fid[1]);
+  }
+
+  // MkBkt (kernel.cu:532-550) as count / scan / scatter / rank / gather.
+  int bucket() {
+    if (nf == 0) {
+      hipLaunchKernelGGL(mps::k_fill_int, dim3(nblk(ncell + 2, 4096)), dim3(mps::kBlock), 0, stream,
+                         fcs, ncell + 2, 0);
+      return MPS_OK;
+    }
+    const dim3 B(mps::kBlock);
+    hipLaunchKernelGGL(mps::k_fill_int, dim3(nblk(ncell + 1, 4096)), B, 0, stream, cnt, ncell + 1, 0);
+    hipLaunchKernelGGL((mps::k_bkt_key<real, DIM>), dim3(nblk(nf)), B, 0, stream, fpos[0],
+                       (const int8_t*)ftyp[0], nf, (const int*)nullptr, c, key, slot, cnt);
+    scan(cnt, ncell + 1, fcs, 0);
+    hipLaunchKernelGGL(mps::k_bkt_scatter, dim3(nblk(nf)), B, 0, stream, key, slot, fcs, nf,
+                       (const int*)nullptr, tmp);
+    hipLaunchKernelGGL(mps::k_bkt_rank, dim3(nblk(nf)), B, 0, stream, tmp, key, fid[0], fcs, nf,
+                       (const int*)nullptr, ncell, perm);
+    hipLaunchKernelGGL((mps::k_gather_fluid<real, DIM>), dim3(nblk(nf)), B, 0, stream, perm, nf,
+                       fa(0), fa(1));
+    swap_all();
+    return MPS_OK;
+  }
+
+  // ResetMRR + GenMRR_nonslip + MkBkt_MRR (kernel.cu:1906-1908).
+  int mirror() {
+    const dim3 B(mps::kBlock);
+    hipLaunchKernelGGL(mps::k_fill_int, dim3(1), B, 0, stream, mcount, 1, 0);
+    if (nf > 0)
+      hipLaunchKernelGGL((mps::k_mirror_gen<real, DIM>), dim3(nblk(nf)), B, 0, stream, fpos[0],
+                         fprs[0], fid[0], fcs, nf, wpos, wnrm, wedge, wcs, c, mraw_pos, mraw_prs,
+                         mraw_id, mcount, mcap);
+    const int g = nblk(mcap, 2048);
+    hipLaunchKernelGGL(mps::k_fill_int, dim3(nblk(ncell + 1, 4096)), B, 0, stream, cnt, ncell + 1, 0);
+    hipLaunchKernelGGL((mps::k_bkt_key<real, DIM>), dim3(g), B, 0, stream, mraw_pos,
+                       (const int8_t*)nullptr, 0, (const int*)mcount, c, key, slot, cnt);
+    scan(cnt, ncell + 1, mcs, 0);
+    hipLaunchKernelGGL(mps::k_bkt_scatter, dim3(g), B, 0, stream, key, slot, mcs, 0,
+                       (const int*)mcount, tmp);
+    hipLaunchKernelGGL(mps::k_bkt_rank, dim3(g), B, 0, stream, tmp, key, mraw_id, mcs, 0,
+                       (const int*)mcount, ncell, perm);
+    hipLaunchKernelGGL((mps::k_gather_mirror<real, DIM>), dim3(g), B, 0, stream, perm,
+                       (const int*)mcount, mraw_pos, mraw_prs, mraw_id, mpos, mprs, mid);
+    return MPS_OK;
+  }
+
+  int check_launch() {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_err(MPS_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+    return MPS_OK;
+  }
+
+  // One ClcMPS loop body (kernel.cu:1904-1935) with fused kernels.  The
+  // redundant MkBkt of kernel.cu:1921 is skipped: positions have not
+  // changed since kernel.cu:1917, so it would rebuild identical buckets.
+  int step(int nsteps) override {
+    const dim3 B(mps::kBlock), G(nblk(nf));
+    if (acc_live) {
+      hipLaunchKernelGGL((mps::k_fill<V>), dim3(nblk(nf, 4096)), B, 0, stream, facc[0], nf,
+                         mps::mkvec<real, DIM>(0, 0, 0));
+      acc_live = false;
+    }
+    for (int s = 0; s < nsteps; s++) {
+      bucket();
+      mirror();
+      if (nf > 0) {
+        hipLaunchKernelGGL((mps::k_vsc<real, DIM, true>), G, B, 0, stream, fpos[0], fvel[0], fcs, nf,
+                           mpos, mcs, c, (V*)nullptr, fpos[1], fvel[1], ftyp[0], fprs[0]);
+        std::swap(fpos[0], fpos[1]);
+        std::swap(fvel[0], fvel[1]);
+      }
+      bucket();
+      if (nf > 0) {
+        hipLaunchKernelGGL((mps::k_col<real, DIM, false>), G, B, 0, stream, fpos[0], fvel[0], fcs, nf,
+                           wpos, wvel, wcs, mpos, mcs, c, fvel[1], (V*)nullptr);
+        std::swap(fvel[0], fvel[1]);
+        hipLaunchKernelGGL((mps::k_prs<real, DIM>), G, B, 0, stream, fpos[0], fcs, nf, mpos, mcs, c,
+                           fprs[0], fnd[0]);
+      }
+      mirror();
+      if (nf > 0) {
+        hipLaunchKernelGGL((mps::k_grad<real, DIM, true>), G, B, 0, stream, fpos[0], fvel[0], fprs[0],
+                           fcs, nf, mpos, mprs, mcs, c, (V*)nullptr, fpos[1], fvel[1], fprs[1],
+                           fpav[0], ftyp[0]);
+        std::swap(fpos[0], fpos[1]);
+        std::swap(fvel[0], fvel[1]);
+        std::swap(fprs[0], fprs[1]);
+      }
+      if (wall_pav && nw > 0)
+        hipLaunchKernelGGL((mps::k_wall_pav<real>), dim3(nblk(nw)), B, 0, stream, wpav, wprs, nw);
+    }
+    return check_launch();
+  }
+
+  int phase(int ph) override {
+    const dim3 B(mps::kBlock), G(nblk(nf));
+    acc_live = true;
+    switch (ph) {
+      case MPS_PH_MKBKT: bucket(); break;
+      case MPS_PH_MIRROR: mirror(); break;
+      case MPS_PH_VSC:
+        if (nf > 0)
+          hipLaunchKernelGGL((mps::k_vsc<real, DIM, false>), G, B, 0, stream, fpos[0], fvel[0], fcs, nf,
+                             mpos, mcs, c, facc[0], (V*)nullptr, (V*)nullptr, (int8_t*)nullptr,
+                             (real*)nullptr);
+        break;
+      case MPS_PH_UP1:
+        if (nf > 0)
+          hipLaunchKernelGGL((mps::k_up1<real, DIM>), G, B, 0, stream, fpos[0], fvel[0], facc[0],
+                             ftyp[0], fprs[0], nf, c);
+        break;
+      case MPS_PH_COL:
+        if (nf > 0) {
+          hipLaunchKernelGGL((mps::k_col<real, DIM, true>), G, B, 0, stream, fpos[0], fvel[0], fcs, nf,
+                             wpos, wvel, wcs, mpos, mcs, c, fvel[1], facc[0]);
+          std::swap(fvel[0], fvel[1]);
+        }
+        break;
+      case MPS_PH_PRS:
+        if (nf > 0)
+          hipLaunchKernelGGL((mps::k_prs<real, DIM>), G, B, 0, stream, fpos[0], fcs, nf, mpos, mcs, c,
+                             fprs[0], fnd[0]);
+        break;
+      case MPS_PH_GRAD:
+        if (nf > 0)
+          hipLaunchKernelGGL((mps::k_grad<real, DIM, false>), G, B, 0, stream, fpos[0], fvel[0],
+                             fprs[0], fcs, nf, mpos, mprs, mcs, c, facc[0], (V*)nullptr, (V*)nullptr,
+                             (real*)nullptr, (real*)nullptr, (int8_t*)nullptr);
+        break;
+      case MPS_PH_UP2:
+        if (nf > 0)
+          hipLaunchKernelGGL((mps::k_up2<real, DIM>), G, B, 0, stream, fpos[0], fvel[0], facc[0],
+                             ftyp[0], fprs[0], fpav[0], nf, c);
+        if (wall_pav && nw > 0)
+          hipLaunchKernelGGL((mps::k_wall_pav<real>), dim3(nblk(nw)), B, 0, stream, wpav, wprs, nw);
+        break;
+      default:
+        return set_err(MPS_E_ARG, "unknown phase " + std::to_string(ph));
+    }
+    return check_launch();
+  }
+
+  template <class T>
+  int d2h(std::vector<T>& h, const T* d, size_t n) {
+    h.resize(n);
+    if (n == 0) return MPS_OK;
+    HIP_TRY(hipMemcpyAsync(h.data(), d, n * sizeof(T), hipMemcpyDeviceToHost, stream));
+    return MPS_OK;
+  }
+
+  int get_state(int8_t* typ, double* pos, double* vel, double* acc, double* prs, double* pav,
+                double* nd) override {
+    std::vector<V> hp, hv, ha, wp, wv;
+    std::vector<real> hprs, hpav, hnd, wprs_h, wpav_h;
+    std::vector<int8_t> ht;
+    std::vector<int> hid;
+    int rc;
+    if ((rc = d2h(hp, fpos[0], nf)) || (rc = d2h(hv, fvel[0], nf)) || (rc = d2h(ha, facc[0], nf)) ||
+        (rc = d2h(hprs, fprs[0], nf)) || (rc = d2h(hpav, fpav[0], nf)) || (rc = d2h(hnd, fnd[0], nf)) ||
+        (rc = d2h(ht, ftyp[0], nf)) || (rc = d2h(hid, fid[0], nf)) || (rc = d2h(wp, wpos, nw)) ||
+        (rc = d2h(wv, wvel, nw)) || (rc = d2h(wprs_h, wprs, nw)) || (rc = d2h(wpav_h, wpav, nw)))
+      return rc;
+    HIP_TRY(hipStreamSynchronize(stream));
+    const size_t n = n_total;
+    auto put = [&](double* a, int i, const V& v) {
+      if (!a) return;
+      a[i] = double(v.x);
+      a[n + i] = double(v.y);
+      a[2 * n + i] = double(mps::vz(v));
+    };
+    for (int f = 0; f < nf; f++) {
+      const int i = hid[f];
+      if (typ) typ[i] = ht[f];
+      put(pos, i, hp[f]);
+      put(vel, i, hv[f]);
+      put(acc, i, acc_live ? ha[f] : mps::mkvec<real, DIM>(0, 0, 0));
+      if (prs) prs[i] = hprs[f];
+      if (pav) pav[i] = hpav[f];
+      if (nd) nd[i] = hnd[f];
+    }
+    for (int w = 0; w < nw; w++) {
+      const int i = host_wall_ids[w];
+      if (typ) typ[i] = wall_typ[w];
+      put(pos, i, wp[w]);
+      put(vel, i, wv[w]);
+      put(acc, i, mps::mkvec<real, DIM>(0, 0, 0));
+      if (prs) prs[i] = wprs_h[w];
+      if (pav) pav[i] = wpav_h[w];
+      if (nd) nd[i] = 0.0;
+    }
+    return MPS_OK;
+  }
+  std::vector<int8_t> wall_typ;
+
+  int set_state(const int8_t* typ, const double* pos, const double* vel, const double* prs,
+                const double* pav) override {
+    std::vector<int> hid;
+    int rc;
+    if ((rc = d2h(hid, fid[0], nf))) return rc;
+    HIP_TRY(hipStreamSynchronize(stream));
+    const size_t n = n_total;
+    if (typ) {
+      for (int i : host_wall_ids)
+        if (!mps::is_wall_type(typ[i])) return set_err(MPS_E_ARG, "the wall set may not change");
+      for (int i : host_fluid_ids)
+        if (typ[i] != MPS_FLD && typ[i] != MPS_GST)
+          return set_err(MPS_E_ARG, "a fluid-set particle may only be FLD or GST");
+    }
+    auto vec_of = [&](const double* a, int i) {
+      return mps::mkvec<real, DIM>(real(a[i]), real(a[n + i]), DIM == 3 ? real(a[2 * n + i]) : real(0));
+    };
+    auto upload = [&](auto* dptr, auto fn) -> int {
+      using T = std::remove_pointer_t<decltype(dptr)>;
+      std::vector<T> h(nf);
+      for (int f = 0; f < nf; f++) h[f] = fn(hid[f]);
+      HIP_TRY(hipMemcpy(dptr, h.data(), nf * sizeof(T), hipMemcpyHostToDevice));
+      return MPS_OK;
+    };
+    if (nf > 0) {
+      if (pos && (rc = upload(fpos[0], [&](int i) { return vec_of(pos, i); }))) return rc;
+      if (vel && (rc = upload(fvel[0], [&](int i) { return vec_of(vel, i); }))) return rc;
+      if (prs && (rc = upload(fprs[0], [&](int i) { return real(prs[i]); }))) return rc;
+      if (pav && (rc = upload(fpav[0], [&](int i) { return real(pav[i]); }))) return rc;
+      if (typ && (rc = upload(ftyp[0], [&](int i) { return int8_t(typ[i]); }))) return rc;
+    }
+    auto wupload = [&](auto* dptr, auto fn) -> int {
+      using T = std::remove_pointer_t<decltype(dptr)>;
+      std::vector<T> h(nw);
+      for (int w = 0; w < nw; w++) h[w] = fn(host_wall_ids[w]);
+      HIP_TRY(hipMemcpy(dptr, h.data(), nw * sizeof(T), hipMemcpyHostToDevice));
+      return MPS_OK;
+    };
+    if (nw > 0) {
+      if (vel && (rc = wupload(wvel, [&](int i) { return vec_of(vel, i); }))) return rc;
+      if (prs) {
+        if ((rc = wupload(wprs, [&](int i) { return real(prs[i]); }))) return rc;
+        wall_pav = false;
+        for (int i : host_wall_ids) wall_pav = wall_pav || prs[i] != 0.0;
+      }
+      if (pav && (rc = wupload(wpav, [&](int i) { return real(pav[i]); }))) return rc;
+      if (typ)
+        for (int w = 0; w < nw; w++) wall_typ[w] = typ[host_wall_ids[w]];
+    }
+    if ((rc = bucket())) return rc;
+    HIP_TRY(hipStreamSynchronize(stream));
+    return check_launch();
+  }
+
+  int buckets(int which, int* cs, int* ids, int* count) override {
+    const int* dcs = which == MPS_BKT_FLUID ? fcs : which == MPS_BKT_WALL ? wcs : mcs;
+    std::vector<int> hcs;
+    int rc;
+    if ((rc = d2h(hcs, dcs, (size_t)ncell + 1))) return rc;
+    HIP_TRY(hipStreamSynchronize(stream));
+    const int n = hcs[ncell];
+    *count = n;
+    if (cs) std::copy(hcs.begin(), hcs.end(), cs);
+    if (ids) {
+      if (which == MPS_BKT_WALL) {
+        for (int k = 0; k < n; k++) ids[k] = host_wall_ids[k];
+      } else {
+        std::vector<int> h;
+        if ((rc = d2h(h, which == MPS_BKT_FLUID ? fid[0] : mid, n))) return rc;
+        HIP_TRY(hipStreamSynchronize(stream));
+        std::copy(h.begin(), h.end(), ids);
+      }
+    }
+    return MPS_OK;
+  }
+
+  int mirrors(int* ids, double* pos, double* prs, int* count) override {
+    int m = 0, rc;
+    if ((rc = buckets(MPS_BKT_MIRROR, nullptr, nullptr, &m))) return rc;
+    *count = m;
+    if (ids || pos || prs) {
+      std::vector<V> hp;
+      std::vector<real> hprs;
+      std::vector<int> hid;
+      if ((rc = d2h(hp, mpos, m)) || (rc = d2h(hprs, mprs, m)) || (rc = d2h(hid, mid, m))) return rc;
+      HIP_TRY(hipStreamSynchronize(stream));
+      for (int k = 0; k < m; k++) {
+        if (ids) ids[k] = hid[k];
+        if (pos) {
+          pos[k] = hp[k].x;
+          pos[(size_t)m + k] = hp[k].y;
+          pos[2 * (size_t)m + k] = mps::vz(hp[k]);
+        }
+        if (prs) prs[k] = hprs[k];
+      }
+    }
+    return MPS_OK;
+  }
+
+  void constants(mps_constants* out) override {
+    mps::fill_constants_out(hk, DIM, out);
+    out->n_fluid = nf;
+    out->n_wall = nw;
+  }
+};
+
+template <class real, int DIM>
+ImplBase* make_impl() {
+  return new Impl<real, DIM>();
+}
+
+}  // namespace
+
+struct mps_handle {
+  std::unique_ptr<ImplBase> impl;
+};
+
+extern "C" {
+
+const char* mps_last_error(void) { return g_err.c_str(); }
+
+int mps_create(const mps_params* p, const mps_scene* s, mps_handle** out) {
+  if (!out) return set_err(MPS_E_ARG, "null output handle");
+  *out = nullptr;
+  std::string err;
+  int rc = mps::validate(p, s, err);
+  if (rc != MPS_OK) return set_err(rc, err);
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return set_err(MPS_E_HIP, "no HIP device available");
+  ImplBase* b;
+  if (p->precision == MPS_FP64) b = p->dim == 3 ? make_impl<double, 3>() : make_impl<double, 2>();
+  else b = p->dim == 3 ? make_impl<float, 3>() : make_impl<float, 2>();
+  std::unique_ptr<mps_handle> h(new mps_handle{std::unique_ptr<ImplBase>(b)});
+  rc = b->init(*p, *s);
+  if (rc != MPS_OK) return rc;
+  *out = h.release();
+  return MPS_OK;
+}
+
+int mps_destroy(mps_handle* h) {
+  delete h;
+  return MPS_OK;
+}
+
+int mps_set_stream(mps_handle* h, void* stream) {
+  if (!h) return set_err(MPS_E_ARG, "null handle");
+  h->impl->stream = (hipStream_t)stream;
+  return MPS_OK;
+}
+
+int mps_sync(mps_handle* h) {
+  if (!h) return set_err(MPS_E_ARG, "null handle");
+  HIP_TRY(hipStreamSynchronize(h->impl->stream));
+  return MPS_OK;
+}
+
+int mps_step(mps_handle* h, int nsteps) {
+  if (!h || nsteps < 0) return set_err(MPS_E_ARG, "bad handle or step count");
+  return h->impl->step(nsteps);
+}
+
+int mps_phase(mps_handle* h, int phase) {
+  if (!h) return set_err(MPS_E_ARG, "null handle");
+  return h->impl->phase(phase);
+}
+
+int mps_get_constants(mps_handle* h, mps_constants* out) {
+  if (!h || !out) return set_err(MPS_E_ARG, "null argument");
+  h->impl->constants(out);
+  return MPS_OK;
+}
+
+int mps_get_state(mps_handle* h, int8_t* typ, double* pos, double* vel, double* acc, double* prs,
+                  double* pav, double* nd) {
+  if (!h) return set_err(MPS_E_ARG, "null handle");
+  return h->impl->get_state(typ, pos, vel, acc, prs, pav, nd);
+}
+
+int mps_set_state(mps_handle* h, const int8_t* typ, const double* pos, const double* vel,
+                  const double* prs, const double* pav) {
+  if (!h) return set_err(MPS_E_ARG, "null handle");
+  return h->impl->set_state(typ, pos, vel, prs, pav);
+}
+
+int mps_get_buckets(mps_handle* h, int which, int* cell_start, int* ids, int* count) {
+  if (!h || !count || which < 0 || which > 2) return set_err(MPS_E_ARG, "bad argument");
+  return h->impl->buckets(which, cell_start, ids, count);
+}
+
+int mps_get_mirrors(mps_handle* h, int* ids, double* pos, double* prs, int* count) {
+  if (!h || !count) return set_err(MPS_E_ARG, "bad argument");
+  return h->impl->mirrors(ids, pos, prs, count);
+}
+
+int mps_compute_constants(const mps_params* p, const mps_scene* s, mps_constants* out) {
+  std::string err;
+  int rc = mps::validate(p, s, err);
+  if (rc != MPS_OK) return set_err(rc, err);
+  if (!out) return set_err(MPS_E_ARG, "null output");
+  int nw = 0;
+  for (int i = 0; i < s->n; i++) nw += mps::is_wall_type(s->typ[i]);
+  if (p->precision == MPS_FP64) {
+    mps::HostConst<double> k;
+    if ((rc = mps::compute_constants<double>(*p, *s, k, err))) return set_err(rc, err);
+    mps::fill_constants_out(k, p->dim, out);
+  } else {
+    mps::HostConst<float> k;
+    if ((rc = mps::compute_constants<float>(*p, *s, k, err))) return set_err(rc, err);
+    mps::fill_constants_out(k, p->dim, out);
+  }
+  out->n_fluid = s->n - nw;
+  out->n_wall = nw;
+  return MPS_OK;
+}
+
+int mps_compute_edge(const mps_params* p, const mps_scene* s, int8_t* edge) {
+  std::string err;
+  int rc = mps::validate(p, s, err);
+  if (rc != MPS_OK) return set_err(rc, err);
+  if (!edge) return set_err(MPS_E_ARG, "null output");
+  std::vector<int8_t> e;
+  if (p->precision == MPS_FP64) mps::compute_edge<double>(*p, *s, e);
+  else mps::compute_edge<float>(*p, *s, e);
+  std::copy(e.begin(), e.end(), edge);
+  return MPS_OK;
+}
+
+}  // extern "C"
+
+template <class real>
+static int probe(const double* a, const double* b, double* so, double* qo, int n) {
+  std::vector<real> ha(n), hb(n), hs(n), hq(n);
+  for (int i = 0; i < n; i++) {
+    ha[i] = real(a[i]);
+    hb[i] = real(b[i]);
+  }
+  real *da = nullptr, *db = nullptr, *ds = nullptr, *dq = nullptr;
+  hipError_t e = hipMalloc(&da, n * sizeof(real));
+  if (e == hipSuccess) e = hipMalloc(&db, n * sizeof(real));
+  if (e == hipSuccess) e = hipMalloc(&ds, n * sizeof(real));
+  if (e == hipSuccess) e = hipMalloc(&dq, n * sizeof(real));
+  if (e == hipSuccess) e = hipMemcpy(da, ha.data(), n * sizeof(real), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(db, hb.data(), n * sizeof(real), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL((mps::k_probe<real>), dim3(nblk(n)), dim3(mps::kBlock), 0, nullptr, da, db, ds,
+                       dq, n);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpy(hs.data(), ds, n * sizeof(real), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(hq.data(), dq, n * sizeof(real), hipMemcpyDeviceToHost);
+  (void)hipFree(da);
+  (void)hipFree(db);
+  (void)hipFree(ds);
+  (void)hipFree(dq);
+  if (e != hipSuccess) return set_err(MPS_E_HIP, std::string("probe: ") + hipGetErrorString(e));
+  for (int i = 0; i < n; i++) {
+    so[i] = hs[i];
+    qo[i] = hq[i];
+  }
+  return MPS_OK;
+}
+
+extern "C" {
+
+int mps_probe_arith(int precision, const double* a, const double* b, double* so, double* qo, int n) {
+  if (!a || !b || !so || !qo || n <= 0) return set_err(MPS_E_ARG, "bad argument");
+  return precision == MPS_FP64 ? probe<double>(a, b, so, qo, n) : probe<float>(a, b, so, qo, n);
+}
+
+}  // extern "C"

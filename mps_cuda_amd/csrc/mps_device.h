@@ -1,0 +1,103 @@
+// mps_device.h -- device-side types and helpers shared by the HIP kernels.
+//
+// Layout in HBM (one GPU):
+//  - fluid set: every particle that is not a wall (FLD, and GST once
+//    deleted), sorted by (bucket cell, particle id) after each bucket build,
+//    deleted particles after all live ones.  Vector fields are packed
+//    (x,y,z,pad) for 3-D and (x,y) for 2-D so that a neighbour is one load.
+//  - wall set: WLL/OBJ/OBJ2 particles, static, sorted once by (cell, id).
+//  - mirror set: live mirror particles of the last mirror phase, sorted by
+//    (cell, mirror id) with mirror id = parent id * 10 + reference slot.
+//  - cell_start arrays of ncell+1 (+1 for the fluid set's deleted tail).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mps {
+
+constexpr int kGST = -1, kFLD = 0, kWLL = 1, kOBJ = 2, kOBJ2 = 3;
+constexpr int kNumMRR = 10;  // const.cuh:50
+constexpr int kEdge = 1;     // const.cuh:36
+constexpr int kBlock = 256;
+
+template <class real, int DIM>
+struct Vec;
+template <>
+struct alignas(16) Vec<float, 3> { float x, y, z, w; };
+template <>
+struct alignas(8) Vec<float, 2> { float x, y; };
+template <>
+struct alignas(32) Vec<double, 3> { double x, y, z, w; };
+template <>
+struct alignas(16) Vec<double, 2> { double x, y; };
+
+template <class real, int DIM>
+__host__ __device__ inline real vz(const Vec<real, DIM>& v) {
+  if constexpr (DIM == 3) return v.z;
+  else return real(0);
+}
+template <class real, int DIM>
+__host__ __device__ inline Vec<real, DIM> mkvec(real x, real y, real z) {
+  Vec<real, DIM> v;
+  v.x = x;
+  v.y = y;
+  if constexpr (DIM == 3) {
+    v.z = z;
+    v.w = real(0);
+  }
+  return v;
+}
+
+// Constants of one scene, computed on the host by SetPara/AlcBkt
+// (kernel.cu:307-417) and passed by value to every kernel.
+template <class real>
+struct Cst {
+  real minc[3], maxc[3], lo[3], hi[3], g[3];
+  real dbinv, l0, r, r2, rp, rp2, rlim2, rlim_wall2;
+  real col, mass, n0knm, grad_a, invro, dt, umax, prs_coef, n0_grad, pmax;
+  int nb[3], nbxy, ncell;
+};
+
+// Bucket cell coordinate, kernel.cu:521-523, clamped to [1, nB-2].
+template <class real>
+__host__ __device__ inline int cell_coord(real x, real minc, real dbinv, int nb) {
+  real t = (x - minc) * dbinv;
+  t = t > real(0) ? t : real(0);  // also maps NaN to 0
+  t = t < real(nb) ? t : real(nb);
+  int c = (int)t + 1;
+  return c < 1 ? 1 : (c > nb - 2 ? nb - 2 : c);
+}
+
+template <class real, int DIM>
+struct Cell {
+  int ix, iy, iz;
+};
+
+template <class real, int DIM>
+__host__ __device__ inline Cell<real, DIM> cell_of(const Vec<real, DIM>& p, const Cst<real>& c) {
+  Cell<real, DIM> k;
+  k.ix = cell_coord(p.x, c.minc[0], c.dbinv, c.nb[0]);
+  k.iy = cell_coord(p.y, c.minc[1], c.dbinv, c.nb[1]);
+  if constexpr (DIM == 3) k.iz = cell_coord(p.z, c.minc[2], c.dbinv, c.nb[2]);
+  else k.iz = 0;
+  return k;
+}
+
+template <class real>
+__host__ __device__ inline int cell_key(int ix, int iy, int iz, const Cst<real>& c) {
+  return iz * c.nbxy + iy * c.nb[0] + ix;
+}
+
+// |q|^2 in the reference's operand order (x, then y, then z).
+template <class real, int DIM>
+__host__ __device__ inline real norm2(real qx, real qy, real qz) {
+  if constexpr (DIM == 3) return qx * qx + qy * qy + qz * qz;
+  else return qx * qx + qy * qy;
+}
+template <class real, int DIM>
+__host__ __device__ inline real dot3(real ax, real ay, real az, real bx, real by, real bz) {
+  if constexpr (DIM == 3) return ax * bx + ay * by + az * bz;
+  else return ax * bx + ay * by;
+}
+
+}  // namespace mps

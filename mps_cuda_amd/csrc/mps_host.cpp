@@ -1,0 +1,368 @@
+// mps_host.cpp -- host-side setup: constants, wall edge flags, scene
+// generation.  No HIP calls: everything here runs without a device.
+#include "mps_host.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <unordered_map>
+
+namespace mps {
+
+int validate(const mps_params* p, const mps_scene* s, std::string& err) {
+  if (!p || !s) return err = "null params or scene", MPS_E_ARG;
+  if (p->dim != 2 && p->dim != 3) return err = "dim must be 2 or 3", MPS_E_ARG;
+  if (p->precision != MPS_FP32 && p->precision != MPS_FP64)
+    return err = "precision must be MPS_FP32 or MPS_FP64", MPS_E_ARG;
+  if (!(p->l0 > 0)) return err = "l0 must be positive", MPS_E_ARG;
+  if (!(p->re_factor > 0) || !(p->rp_factor > 0) || p->rp_factor > p->re_factor)
+    return err = "need 0 < rp_factor <= re_factor", MPS_E_ARG;
+  if (!(p->c > 0) && !(p->mach > 0)) return err = "need c > 0 or mach > 0", MPS_E_ARG;
+  if (s->n <= 0 || !s->typ || !s->pos) return err = "empty scene", MPS_E_ARG;
+  bool walls = false;
+  for (int i = 0; i < s->n; i++) {
+    const int t = s->typ[i];
+    if (t != MPS_GST && t != MPS_FLD && !is_wall_type(t))
+      return err = "particle " + std::to_string(i) + ": bad type " + std::to_string(t), MPS_E_ARG;
+    walls = walls || is_wall_type(t);
+  }
+  if (walls && !s->normal) return err = "walls need normals", MPS_E_ARG;
+  return MPS_OK;
+}
+
+template <class real>
+int compute_constants(const mps_params& p, const mps_scene& s, HostConst<real>& k,
+                      std::string& err) {
+  const int dim = p.dim, n = s.n;
+  k.l0 = real(p.l0);
+  for (int a = 0; a < 3; a++) {
+    // kernel.cu:26-31
+    k.minc[a] = real(p.range_min[a] - p.re_factor * double(k.l0));
+    k.maxc[a] = real(p.range_max[a] + p.re_factor * double(k.l0));
+    k.g[a] = real(p.gravity[a]);
+  }
+  if (dim == 2) k.minc[2] = k.maxc[2] = k.g[2] = real(0);
+  // AlcBkt, kernel.cu:309-322
+  k.r = k.l0 * real(p.re_factor);
+  k.r2 = k.r * k.r;
+  k.rp = k.l0 * real(p.rp_factor);
+  k.rp2 = k.rp * k.rp;
+  k.db = k.l0 * real(p.re_factor);
+  k.dbinv = real(1) / k.db;
+  for (int a = 0; a < 3; a++) {
+    if (a < dim) {
+      const real span = (k.maxc[a] - k.minc[a]) * k.dbinv;
+      if (!(span > 0) || span > real(1 << 24)) return err = "domain range invalid", MPS_E_ARG;
+      k.nb[a] = (int)span + 3;
+    } else {
+      k.nb[a] = 1;
+    }
+  }
+  k.nbxy = k.nb[0] * k.nb[1];
+  const long long ncell = (long long)k.nbxy * k.nb[2];
+  if (ncell > (1ll << 30)) return err = "bucket grid too large", MPS_E_ARG;
+  k.ncell = (int)ncell;
+  for (int i = 0; i < n; i++) {
+    if (s.typ[i] == MPS_GST) continue;
+    for (int a = 0; a < dim; a++) {
+      const real x = real(s.pos[a * (size_t)n + i]);
+      if (!(x > k.minc[a] && x < k.maxc[a]))
+        return err = "particle " + std::to_string(i) + " lies outside the domain", MPS_E_ARG;
+    }
+  }
+  // SetPara lattice sums, kernel.cu:336-366
+  real tn0 = 0, tn0_grad = 0, tlmd = 0, tlmd_grad = 0;
+  int tnum = 0;
+  const int zlo = dim == 3 ? -10 : 0, zhi = dim == 3 ? 10 : 1;
+  for (int ix = -10; ix < 10; ix++)
+    for (int iy = -10; iy < 10; iy++)
+      for (int iz = zlo; iz < zhi; iz++) {
+        const real x = k.l0 * real(ix), y = k.l0 * real(iy), z = k.l0 * real(iz);
+        const real dist2 = x * x + y * y + z * z;
+        if (dist2 == real(0)) continue;
+        const real dist = std::sqrt(dist2);
+        if (dist2 < k.rp2) {
+          const real w = (k.rp / dist) - (dist / k.rp);
+          tn0_grad += w;
+          tlmd_grad += dist2 * w;
+          tnum += 1;
+        }
+        if (dist2 < k.r2) {
+          const real w = (k.r / dist) + (dist / k.r) - real(2);
+          tn0 += w;
+          tlmd += dist2 * w;
+        }
+      }
+  k.n0 = tn0;
+  k.lmd = tlmd / tn0;
+  k.n0_grad = tn0_grad;
+  k.lmd_grad = tlmd_grad / tn0_grad;
+  k.num_n0_grad = tnum;
+  // kernel.cu:370-373, 1184, 484
+  k.rlim = k.l0 * real(p.dst_lmt_rat);
+  k.rlim2 = k.rlim * k.rlim;
+  k.col = real(p.restitution) + real(1);
+  k.mass = real(p.rho);
+  k.nu = real(p.nu);
+  k.rlim_wall = real(p.rlim_wall_factor) * k.l0;
+  k.rlim_wall2 = k.rlim_wall * k.rlim_wall;
+  k.inset = real(p.re_factor) * k.l0;
+  // kernel.cu:399-415 (the if / else-if height scan is kept as written)
+  real hmax = k.minc[1], hmin = k.maxc[1];
+  for (int i = 0; i < n; i++) {
+    if (s.typ[i] != MPS_FLD) continue;
+    const real y = real(s.pos[(size_t)n + i]);
+    if (hmax < y) hmax = y;
+    else if (hmin > y) hmin = y;
+  }
+  k.max_height = hmax;
+  k.min_height = hmin;
+  k.umax = std::sqrt(real(2) * std::fabs(k.g[1]) * (hmax - hmin));
+  k.snd = p.c > 0 ? real(p.c) : k.umax / real(p.mach);
+  k.prs_coef = k.snd * k.snd / k.n0_grad;
+  if (p.pmax_mode == MPS_PMAX_ABS_G) k.pmax = k.mass * std::fabs(k.g[1]) * (hmax - hmin);
+  else if (p.pmax_mode == MPS_PMAX_REFERENCE) k.pmax = k.mass * k.g[1] * (hmax - hmin);
+  else k.pmax = real(INFINITY);
+  k.dt = p.dt > 0 ? real(p.dt) : real(k.l0 / (k.umax + k.snd));
+  return MPS_OK;
+}
+
+template int compute_constants<float>(const mps_params&, const mps_scene&, HostConst<float>&,
+                                      std::string&);
+template int compute_constants<double>(const mps_params&, const mps_scene&, HostConst<double>&,
+                                       std::string&);
+
+template <class real>
+void compute_edge(const mps_params& p, const mps_scene& s, std::vector<int8_t>& edge) {
+  const int n = s.n, dim = p.dim;
+  edge.assign(n, MPS_SURFACE);
+  const double R = p.edge_factor * p.l0, R2 = R * R;
+  auto coord = [&](int i, int a) { return s.pos[a * (size_t)n + i]; };
+  auto ck = [&](int i, int a) { return (long long)std::floor(coord(i, a) / R); };
+  auto hk = [](long long x, long long y, long long z) {
+    return ((unsigned long long)(x + (1 << 20)) << 42) ^ ((unsigned long long)(y + (1 << 20)) << 21) ^
+           (unsigned long long)(z + (1 << 20));
+  };
+  auto nrm = [&](int i, int a) { return a < dim ? real(s.normal[a * (size_t)n + i]) : real(0); };
+  std::unordered_map<unsigned long long, std::vector<int>> grid;
+  for (int i = 0; i < n; i++)
+    if (is_wall_type(s.typ[i])) grid[hk(ck(i, 0), ck(i, 1), dim == 3 ? ck(i, 2) : 0)].push_back(i);
+  for (int i = 0; i < n; i++) {
+    if (!is_wall_type(s.typ[i])) continue;
+    const long long cx = ck(i, 0), cy = ck(i, 1), cz = dim == 3 ? ck(i, 2) : 0;
+    const int zr = dim == 3 ? 1 : 0;
+    for (long long dz = -zr; dz <= zr; dz++)
+      for (long long dy = -1; dy <= 1; dy++)
+        for (long long dx = -1; dx <= 1; dx++) {
+          auto it = grid.find(hk(cx + dx, cy + dy, cz + dz));
+          if (it == grid.end()) continue;
+          for (int j : it->second) {
+            if (j == i || s.typ[j] != s.typ[i]) continue;
+            const double px = coord(i, 0) - coord(j, 0);
+            const double py = coord(i, 1) - coord(j, 1);
+            const double pz = dim == 3 ? coord(i, 2) - coord(j, 2) : 0.0;
+            if (px * px + py * py + pz * pz < R2 &&
+                (nrm(j, 0) != nrm(i, 0) || nrm(j, 1) != nrm(i, 1) || nrm(j, 2) != nrm(i, 2)))
+              edge[i] = MPS_EDGE;
+          }
+        }
+  }
+}
+
+template void compute_edge<float>(const mps_params&, const mps_scene&, std::vector<int8_t>&);
+template void compute_edge<double>(const mps_params&, const mps_scene&, std::vector<int8_t>&);
+
+template <class real>
+static void fill_out(const HostConst<real>& k, int dim, mps_constants* c) {
+  std::memset(c, 0, sizeof(*c));
+  c->n0 = k.n0;
+  c->lmd = k.lmd;
+  c->n0_grad = k.n0_grad;
+  c->lmd_grad = k.lmd_grad;
+  c->num_n0_grad = k.num_n0_grad;
+  for (int a = 0; a < 3; a++) {
+    c->nb[a] = k.nb[a];
+    c->min_c[a] = k.minc[a];
+    c->max_c[a] = k.maxc[a];
+  }
+  c->ncell = k.ncell;
+  c->r = k.r;
+  c->rp = k.rp;
+  c->rlim = k.rlim;
+  c->rlim_wall = k.rlim_wall;
+  c->db = k.db;
+  c->dbinv = k.dbinv;
+  c->umax = k.umax;
+  c->snd = k.snd;
+  c->prs_coef = k.prs_coef;
+  c->pmax = k.pmax;
+  c->dt = k.dt;
+  c->col = k.col;
+  c->max_height = k.max_height;
+  c->min_height = k.min_height;
+  (void)dim;
+}
+void fill_constants_out(const HostConst<double>& k, int dim, mps_constants* out) {
+  fill_out(k, dim, out);
+}
+void fill_constants_out(const HostConst<float>& k, int dim, mps_constants* out) {
+  fill_out(k, dim, out);
+}
+
+}  // namespace mps
+
+// ---------------------------------------------------------------------------
+// Scene generation.
+
+namespace {
+
+// SplitMix64 (Steele, Lea, Flood 2014).
+struct SplitMix64 {
+  uint64_t s;
+  uint64_t next() {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  }
+  double uniform() { return double(next() >> 11) * 0x1.0p-53; }
+};
+
+// Jitter values are odd multiples of half a quantum (2^-19 m), so no fluid
+// particle sits within rounding distance of the lattice plane it was
+// displaced from: mirror culling (kernel.cu:982-983) then never hinges on
+// the last bit of a coordinate.
+constexpr double kJitterQuantum = 0x1.0p-19;
+
+struct Dims {
+  long long nx, ny, nz, fx, fy, fz;
+};
+
+Dims dims_of(const mps_dambreak_spec* s) {
+  Dims d;
+  d.nx = std::llround(s->tank[0] / s->l0);
+  d.ny = std::llround(s->tank[1] / s->l0);
+  d.nz = s->dim == 3 ? std::llround(s->tank[2] / s->l0) : 1;
+  d.fx = std::llround(s->column[0] / s->l0);
+  d.fy = std::llround(s->column[1] / s->l0);
+  d.fz = s->dim == 3 ? std::llround(s->column[2] / s->l0) : 1;
+  return d;
+}
+
+bool spec_ok(const mps_dambreak_spec* s) {
+  if (!s || (s->dim != 2 && s->dim != 3) || !(s->l0 > 0) || s->wall_layers < 0) return false;
+  Dims d = dims_of(s);
+  return d.nx > 0 && d.ny > 0 && d.nz > 0 && d.fx > 0 && d.fy > 0 && d.fz > 0 && d.fx <= d.nx &&
+         d.fy <= d.ny && d.fz <= d.nz;
+}
+
+// Wall sites in layer order: the innermost shell first, then the next.
+// d_GenMRR_nonslip reflects a fluid particle across the plane of the FIRST
+// wall particle it meets with a given normal (kernel.cu:767-774); with ids
+// growing outward, the first one met inside a bucket is an innermost one.
+template <class F>
+void for_each_wall(const mps_dambreak_spec* s, const Dims& d, F&& f) {
+  const long long L = s->wall_layers;
+  const long long klo = s->dim == 3 ? -L : 0, khi = s->dim == 3 ? d.nz + L : 1;
+  auto depth = [&](long long i, long long j, long long k) {
+    long long t = -1;
+    if (i < 0) t = std::max(t, -1 - i);
+    if (i >= d.nx) t = std::max(t, i - d.nx);
+    if (j < 0) t = std::max(t, -1 - j);
+    if (s->dim == 3 && k < 0) t = std::max(t, -1 - k);
+    if (s->dim == 3 && k >= d.nz) t = std::max(t, k - d.nz);
+    return t;
+  };
+  for (long long layer = 0; layer < L; layer++)
+    for (long long k = klo; k < khi; k++)
+      for (long long j = -L; j < d.ny; j++)
+        for (long long i = -L; i < d.nx + L; i++)
+          if (depth(i, j, k) == layer) f(i, j, k);
+}
+
+}  // namespace
+
+extern "C" int mps_scene_dambreak_count(const mps_dambreak_spec* s, int* n_fluid, int* n_wall) {
+  if (!spec_ok(s) || !n_fluid || !n_wall) return MPS_E_ARG;
+  Dims d = dims_of(s);
+  long long nf = d.fx * d.fy * d.fz, nw = 0;
+  for_each_wall(s, d, [&](long long, long long, long long) { nw++; });
+  if (nf + nw > (1ll << 30)) return MPS_E_ARG;
+  *n_fluid = (int)nf;
+  *n_wall = (int)nw;
+  return MPS_OK;
+}
+
+extern "C" int mps_scene_dambreak_fill(const mps_dambreak_spec* s, int8_t* typ, double* pos,
+                                       double* normal, double* range_min, double* range_max) {
+  int nf, nw;
+  if (mps_scene_dambreak_count(s, &nf, &nw) != MPS_OK || !typ || !pos) return MPS_E_ARG;
+  const Dims d = dims_of(s);
+  const size_t n = (size_t)nf + nw;
+  const double l0 = s->l0, J = s->jitter * l0;
+  SplitMix64 rng{s->seed};
+  auto jit = [&]() {
+    if (!(J > 0)) return 0.0;
+    const double u = -J + 2.0 * J * rng.uniform();
+    return (std::floor(u / kJitterQuantum) + 0.5) * kJitterQuantum;
+  };
+  size_t p = 0;
+  for (long long k = 0; k < d.fz; k++)
+    for (long long j = 0; j < d.fy; j++)
+      for (long long i = 0; i < d.fx; i++, p++) {
+        typ[p] = MPS_FLD;
+        pos[p] = (i + 0.5) * l0 + jit();
+        pos[n + p] = (j + 0.5) * l0 + jit();
+        pos[2 * n + p] = s->dim == 3 ? (k + 0.5) * l0 + jit() : 0.0;
+        if (normal) normal[p] = normal[n + p] = normal[2 * n + p] = 0.0;
+      }
+  for_each_wall(s, d, [&](long long i, long long j, long long k) {
+    typ[p] = MPS_WLL;
+    pos[p] = (i + 0.5) * l0;
+    pos[n + p] = (j + 0.5) * l0;
+    pos[2 * n + p] = s->dim == 3 ? (k + 0.5) * l0 : 0.0;
+    // inward normal: sum of the normals of the faces the site lies beyond
+    double nx = (i < 0 ? 1.0 : 0.0) - (i >= d.nx ? 1.0 : 0.0);
+    double ny = j < 0 ? 1.0 : 0.0;
+    double nz = s->dim == 3 ? (k < 0 ? 1.0 : 0.0) - (k >= d.nz ? 1.0 : 0.0) : 0.0;
+    const double len = std::sqrt(nx * nx + ny * ny + nz * nz);
+    if (normal) {
+      normal[p] = nx / len;
+      normal[n + p] = ny / len;
+      normal[2 * n + p] = nz / len;
+    }
+    p++;
+  });
+  // The range is offset by 0.05 l0 from the lattice so that no lattice site
+  // (and no mirror of one) lies on a bucket boundary.
+  const double L = s->wall_layers + 0.05;
+  if (range_min) {
+    range_min[0] = -L * l0;
+    range_min[1] = -L * l0;
+    range_min[2] = s->dim == 3 ? -L * l0 : 0.0;
+  }
+  if (range_max) {
+    range_max[0] = (d.nx + L) * l0;
+    range_max[1] = s->domain_top > 0 ? s->domain_top : (d.ny + L) * l0;
+    range_max[2] = s->dim == 3 ? (d.nz + L) * l0 : 0.0;
+  }
+  return MPS_OK;
+}
+
+extern "C" int mps_params_default(mps_params* p) {
+  if (!p) return MPS_E_ARG;
+  std::memset(p, 0, sizeof(*p));
+  p->dim = 3;
+  p->precision = MPS_FP32;
+  p->pmax_mode = MPS_PMAX_NONE;
+  p->rho = 1000.0;          // const.cuh:38-41
+  p->re_factor = 3.1;       // kernel.cu:309, 313
+  p->rp_factor = 2.1;       // kernel.cu:311
+  p->rlim_wall_factor = 0.45;  // kernel.cu:1184
+  p->edge_factor = 1.01;
+  p->dst_lmt_rat = 0.5;
+  p->restitution = 0.2;
+  p->mach = 0.1;
+  return MPS_OK;
+}

@@ -1,0 +1,37 @@
+// mps_host.h -- host-side setup of a scene (no device calls).
+#pragma once
+#include <string>
+#include <vector>
+
+#include "../../include/mps.h"
+
+namespace mps {
+
+// Host constants of one scene in the handle's precision.
+template <class real>
+struct HostConst {
+  real minc[3], maxc[3], g[3];
+  real l0, r, r2, rp, rp2, db, dbinv, rlim, rlim2, rlim_wall, rlim_wall2, inset, col, mass, nu;
+  real n0, lmd, n0_grad, lmd_grad, umax, snd, prs_coef, pmax, dt, max_height, min_height;
+  int num_n0_grad, nb[3], nbxy, ncell;
+};
+
+// AlcBkt + SetPara (kernel.cu:26-31, 307-417) with the generalisations of
+// mps_params.  Returns MPS_OK or an MPS_E_* code with a message in `err`.
+template <class real>
+int compute_constants(const mps_params& p, const mps_scene& s, HostConst<real>& k,
+                      std::string& err);
+
+// d_Surface_Edge (kernel.cu:553-668) once on the host in FP64 with radius
+// edge_factor*l0; normals are compared after rounding to `real`.
+template <class real>
+void compute_edge(const mps_params& p, const mps_scene& s, std::vector<int8_t>& edge);
+
+void fill_constants_out(const HostConst<double>& k, int dim, mps_constants* out);
+void fill_constants_out(const HostConst<float>& k, int dim, mps_constants* out);
+
+int validate(const mps_params* p, const mps_scene* s, std::string& err);
+
+inline bool is_wall_type(int t) { return t == MPS_WLL || t == MPS_OBJ || t == MPS_OBJ2; }
+
+}  // namespace mps

@@ -1,0 +1,304 @@
+"""ctypes binding of the C ABI declared in include/mps.h.
+
+`Handle` drives one simulation through a library that exports the handle
+functions of include/mps.h under a prefix ("mps_" for the HIP library).  The
+product library is ``libmps_hip.so`` next to this file; importing a handle
+without it raises instead of falling back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmps_hip.so")
+
+GST, FLD, WLL, OBJ, OBJ2, MRR = -1, 0, 1, 2, 3, 4
+SURFACE, EDGE = 0, 1
+NUM_MRR = 10
+FP32, FP64 = 0, 1
+PMAX_NONE, PMAX_ABS_G, PMAX_REFERENCE = 0, 1, 2
+PH_MKBKT, PH_MIRROR, PH_VSC, PH_UP1, PH_COL, PH_PRS, PH_GRAD, PH_UP2 = range(8)
+BKT_FLUID, BKT_WALL, BKT_MIRROR = 0, 1, 2
+STEP_PHASES = [PH_MKBKT, PH_MIRROR, PH_VSC, PH_UP1, PH_MKBKT, PH_COL, PH_MKBKT, PH_PRS,
+               PH_MIRROR, PH_GRAD, PH_UP2]  # MPS::ClcMPS, kernel.cu:1904-1935
+
+_d3 = C.c_double * 3
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int), ("precision", C.c_int), ("pmax_mode", C.c_int), ("reserved0", C.c_int),
+        ("l0", C.c_double), ("range_min", _d3), ("range_max", _d3), ("gravity", _d3),
+        ("rho", C.c_double), ("nu", C.c_double), ("c", C.c_double), ("mach", C.c_double),
+        ("dt", C.c_double), ("restitution", C.c_double), ("dst_lmt_rat", C.c_double),
+        ("re_factor", C.c_double), ("rp_factor", C.c_double), ("rlim_wall_factor", C.c_double),
+        ("edge_factor", C.c_double),
+    ]
+
+
+class Scene(C.Structure):
+    _fields_ = [
+        ("n", C.c_int), ("reserved0", C.c_int), ("typ", C.c_void_p), ("pos", C.c_void_p),
+        ("vel", C.c_void_p), ("normal", C.c_void_p), ("prs", C.c_void_p), ("pav", C.c_void_p),
+        ("edge", C.c_void_p),
+    ]
+
+
+class Constants(C.Structure):
+    _fields_ = [
+        ("n0", C.c_double), ("lmd", C.c_double), ("n0_grad", C.c_double), ("lmd_grad", C.c_double),
+        ("num_n0_grad", C.c_int), ("nb", C.c_int * 3), ("ncell", C.c_int), ("n_fluid", C.c_int),
+        ("n_wall", C.c_int), ("reserved0", C.c_int),
+        ("r", C.c_double), ("rp", C.c_double), ("rlim", C.c_double), ("rlim_wall", C.c_double),
+        ("db", C.c_double), ("dbinv", C.c_double), ("min_c", _d3), ("max_c", _d3),
+        ("umax", C.c_double), ("snd", C.c_double), ("prs_coef", C.c_double), ("pmax", C.c_double),
+        ("dt", C.c_double), ("col", C.c_double), ("max_height", C.c_double),
+        ("min_height", C.c_double),
+    ]
+
+    def as_dict(self):
+        out = {}
+        for name, _ in self._fields_:
+            if name.startswith("reserved"):
+                continue
+            v = getattr(self, name)
+            out[name] = list(v) if hasattr(v, "__len__") else v
+        return out
+
+
+class DamBreakSpec(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int), ("wall_layers", C.c_int), ("seed", C.c_uint64), ("l0", C.c_double),
+        ("tank", _d3), ("column", _d3), ("jitter", C.c_double), ("domain_top", C.c_double),
+    ]
+
+
+_LIB = None
+
+
+def load_library(path: str = LIB_PATH) -> C.CDLL:
+    """Loads the HIP library; raises if it has not been built."""
+    global _LIB
+    if _LIB is not None and path == LIB_PATH:
+        return _LIB
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with `python -m mps_cuda_amd.build`")
+    lib = C.CDLL(path)
+    _declare(lib, "mps_")
+    lib.mps_last_error.restype = C.c_char_p
+    for fn in ("mps_scene_dambreak_count", "mps_scene_dambreak_fill", "mps_params_default",
+               "mps_compute_constants", "mps_compute_edge", "mps_probe_arith", "mps_sync",
+               "mps_set_stream"):
+        getattr(lib, fn).restype = C.c_int
+    if path == LIB_PATH:
+        _LIB = lib
+    return lib
+
+
+def _declare(lib, prefix):
+    vp = C.c_void_p
+    sig = {
+        "create": [vp, vp, C.POINTER(vp)], "destroy": [vp], "step": [vp, C.c_int],
+        "phase": [vp, C.c_int], "get_constants": [vp, vp],
+        "get_state": [vp] * 8, "set_state": [vp] * 6,
+        "get_buckets": [vp, C.c_int, vp, vp, vp], "get_mirrors": [vp, vp, vp, vp, vp],
+    }
+    for name, args in sig.items():
+        f = getattr(lib, prefix + name)
+        f.argtypes = args
+        f.restype = C.c_int
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+def default_params(**kw) -> Params:
+    p = Params()
+    load_library().mps_params_default(C.byref(p))
+    for k, v in kw.items():
+        if k == "precision" and isinstance(v, str):
+            v = FP64 if v.lower() in ("fp64", "f64", "double") else FP32
+        cur = getattr(p, k)
+        if hasattr(cur, "__len__"):
+            for i, x in enumerate(v):
+                cur[i] = x
+        else:
+            setattr(p, k, v)
+    return p
+
+
+@dataclass
+class SceneArrays:
+    typ: np.ndarray          # int8 [n]
+    pos: np.ndarray          # float64 [3, n]
+    normal: np.ndarray       # float64 [3, n]
+    vel: np.ndarray | None = None
+    prs: np.ndarray | None = None
+    pav: np.ndarray | None = None
+    edge: np.ndarray | None = None
+    range_min: tuple = (0.0, 0.0, 0.0)
+    range_max: tuple = (0.0, 0.0, 0.0)
+
+    @property
+    def n(self):
+        return int(self.typ.shape[0])
+
+    def struct(self) -> Scene:
+        s = Scene()
+        s.n = self.n
+        self._keep = []
+        for name in ("typ", "pos", "vel", "normal", "prs", "pav", "edge"):
+            a = getattr(self, name)
+            if a is not None:
+                dt = np.int8 if name in ("typ", "edge") else np.float64
+                a = np.ascontiguousarray(a, dtype=dt)
+                self._keep.append(a)
+                setattr(s, name, a.ctypes.data)
+        return s
+
+
+def dambreak(dim=2, l0=0.01, tank=(1.0, 0.6, 0.0), column=(0.25, 0.5, 0.0), wall_layers=3,
+             jitter=0.0, seed=0, domain_top=1.0) -> SceneArrays:
+    """Dam-break scene from the library's generator (fluid first, then walls)."""
+    lib = load_library()
+    spec = DamBreakSpec()
+    spec.dim, spec.wall_layers, spec.seed, spec.l0 = dim, wall_layers, seed, l0
+    for i in range(3):
+        spec.tank[i] = tank[i]
+        spec.column[i] = column[i]
+    spec.jitter, spec.domain_top = jitter, domain_top
+    nf, nw = C.c_int(), C.c_int()
+    rc = lib.mps_scene_dambreak_count(C.byref(spec), C.byref(nf), C.byref(nw))
+    if rc:
+        raise ValueError("invalid dam-break specification")
+    n = nf.value + nw.value
+    typ = np.zeros(n, np.int8)
+    pos = np.zeros((3, n), np.float64)
+    nrm = np.zeros((3, n), np.float64)
+    rmin, rmax = (C.c_double * 3)(), (C.c_double * 3)()
+    rc = lib.mps_scene_dambreak_fill(C.byref(spec), typ.ctypes.data_as(C.c_void_p),
+                                     pos.ctypes.data_as(C.c_void_p), nrm.ctypes.data_as(C.c_void_p),
+                                     rmin, rmax)
+    if rc:
+        raise ValueError("dam-break generation failed")
+    return SceneArrays(typ=typ, pos=pos, normal=nrm, range_min=tuple(rmin), range_max=tuple(rmax))
+
+
+class MPSError(RuntimeError):
+    pass
+
+
+class Handle:
+    """One simulation behind a C ABI with the handle functions of mps.h."""
+
+    def __init__(self, params: Params, scene: SceneArrays, lib=None, prefix="mps_"):
+        self.lib = lib if lib is not None else load_library()
+        self.prefix = prefix
+        self.params = params
+        self.n = scene.n
+        self.dim = params.dim
+        self._h = C.c_void_p()
+        s = scene.struct()
+        self._check(self._f("create")(C.byref(params), C.byref(s), C.byref(self._h)))
+
+    def _f(self, name):
+        return getattr(self.lib, self.prefix + name)
+
+    def _check(self, rc):
+        if rc != 0:
+            msg = getattr(self.lib, self.prefix + "last_error")()
+            raise MPSError(f"{self.prefix}: status {rc}: {msg.decode() if msg else ''}")
+
+    def close(self):
+        if self._h:
+            self._f("destroy")(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, n=1):
+        self._check(self._f("step")(self._h, n))
+
+    def phase(self, ph):
+        self._check(self._f("phase")(self._h, ph))
+
+    def constants(self) -> dict:
+        c = Constants()
+        self._check(self._f("get_constants")(self._h, C.byref(c)))
+        return c.as_dict()
+
+    def state(self) -> dict:
+        n = self.n
+        typ = np.zeros(n, np.int8)
+        pos, vel, acc = (np.zeros((3, n)) for _ in range(3))
+        prs, pav, nd = (np.zeros(n) for _ in range(3))
+        self._check(self._f("get_state")(self._h, *(a.ctypes.data for a in (typ, pos, vel, acc, prs, pav, nd))))
+        return dict(typ=typ, pos=pos, vel=vel, acc=acc, prs=prs, pav=pav, nd=nd)
+
+    def set_state(self, typ=None, pos=None, vel=None, prs=None, pav=None):
+        arrs = []
+        for a, dt in ((typ, np.int8), (pos, np.float64), (vel, np.float64), (prs, np.float64),
+                      (pav, np.float64)):
+            arrs.append(None if a is None else np.ascontiguousarray(a, dtype=dt))
+        self._check(self._f("set_state")(self._h, *(_ptr(a) for a in arrs)))
+
+    def buckets(self, which=BKT_FLUID):
+        cnt = C.c_int()
+        ncell = self.constants()["ncell"]
+        cs = np.zeros(ncell + 1, np.int32)
+        self._check(self._f("get_buckets")(self._h, which, cs.ctypes.data, None, C.byref(cnt)))
+        ids = np.zeros(max(cnt.value, 1), np.int32)
+        self._check(self._f("get_buckets")(self._h, which, cs.ctypes.data, ids.ctypes.data, C.byref(cnt)))
+        return cs, ids[: cnt.value]
+
+    def mirrors(self):
+        cnt = C.c_int()
+        self._check(self._f("get_mirrors")(self._h, None, None, None, C.byref(cnt)))
+        m = cnt.value
+        ids = np.zeros(max(m, 1), np.int32)
+        pos = np.zeros((3, max(m, 1)))
+        prs = np.zeros(max(m, 1))
+        self._check(self._f("get_mirrors")(self._h, ids.ctypes.data, pos.ctypes.data, prs.ctypes.data, C.byref(cnt)))
+        return ids[:m], pos[:, :m] if m else np.zeros((3, 0)), prs[:m]
+
+
+def compute_constants(params: Params, scene: SceneArrays) -> dict:
+    lib = load_library()
+    c = Constants()
+    s = scene.struct()
+    rc = lib.mps_compute_constants(C.byref(params), C.byref(s), C.byref(c))
+    if rc:
+        raise MPSError(lib.mps_last_error().decode())
+    return c.as_dict()
+
+
+def compute_edge(params: Params, scene: SceneArrays) -> np.ndarray:
+    lib = load_library()
+    e = np.zeros(scene.n, np.int8)
+    s = scene.struct()
+    rc = lib.mps_compute_edge(C.byref(params), C.byref(s), e.ctypes.data_as(C.c_void_p))
+    if rc:
+        raise MPSError(lib.mps_last_error().decode())
+    return e
+
+
+def probe_arith(precision, a, b):
+    lib = load_library()
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    s = np.zeros_like(a)
+    q = np.zeros_like(a)
+    rc = lib.mps_probe_arith(precision, C.c_void_p(a.ctypes.data), C.c_void_p(b.ctypes.data),
+                             C.c_void_p(s.ctypes.data), C.c_void_p(q.ctypes.data), len(a))
+    if rc:
+        raise MPSError(lib.mps_last_error().decode())
+    return s, q

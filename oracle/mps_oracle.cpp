@@ -1,0 +1,970 @@
+/*
+ * mps_oracle.cpp -- CPU oracle of the explicit MPS time step.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP path and the CPU
+ * baseline timed beside it.  Nothing in the product links or calls this file.
+ *
+ * It restates the reference's loop bodies (kernel.cu of frsykzk/MPS_CUDA)
+ * one for one, on the reference's own data structures: per-cell singly
+ * linked lists (bfst/blst/nxt) over all particles, and NumMRR = 10 mirror
+ * slots per particle (PosM/VelM/PrsM/TypM/FromWLL).  Each function cites the
+ * lines it follows.  It is templated on `real` (double = FP64 oracle, float =
+ * FP32 instantiation) and runs the per-particle loops with OpenMP.
+ *
+ * Decisions where the reference is undefined or hard-coded (numbered as the
+ * reference quirks H1..H18 in DESIGN.md):
+ *  - In-cell order (H4).  The reference appends with atomicExch, so the order
+ *    inside a cell is not defined.  Here a cell's list holds its fluid
+ *    particles in ascending index, then its wall particles in ascending index
+ *    (MkBkt below).  Mirror lists hold slots in ascending slot index.
+ *  - dim (H2): lattice sums over `dim` axes and A = dim/n0_grad.  In 2-D the
+ *    z coordinate is absent (the reference's z loop then only meets empty
+ *    cells, and every z term is an exact zero), and ChkPcl tests x and y.
+ *  - c and dt are inputs (H3); Pmax follows pmax_mode (H1).
+ *  - The gradient's mirror loop has no `j != i` test (H5); synvec starts at 0
+ *    (H6); the edge mirror is skipped when no slot is free (H7); WLLSE is
+ *    computed once in FP64 with radius edge_factor*l0 (H8).
+ *  - Cell coordinates are clamped to [1, nB-2] (identical to the reference
+ *    for every particle inside the domain; the reference would index out of
+ *    bounds otherwise).
+ *  - d_MRRinout is skipped for empty (GST) slots: it cannot change them.
+ *
+ * Arithmetic: plain IEEE operations in the reference's order, built with
+ * -ffp-contract=off; the only library call is sqrt (correctly rounded).
+ */
+#include "mps_oracle.h"
+
+#include <omp.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+namespace {
+
+constexpr int GST = -1, FLD = 0, WLL = 1, OBJ = 2, OBJ2 = 3, MRR = 4;  // const.cuh:28-33
+constexpr int NumMRR = 10;                                             // const.cuh:50
+constexpr int Surface = 0, Edge = 1;                                   // const.cuh:35-36
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+inline bool is_wall(int t) { return t == WLL || t == OBJ || t == OBJ2; }
+
+// const.cuh:45-46
+template <class real>
+inline real WEI(real dst, real re) { return (re / dst) + (dst / re) - real(2); }
+template <class real>
+inline real WEI_grad(real dst, real re) { return (re / dst) - (dst / re); }
+
+struct Base {
+  virtual ~Base() {}
+  virtual int step(int n) = 0;
+  virtual int phase(int p) = 0;
+  virtual void constants(mps_constants* c) = 0;
+  virtual void get_state(int8_t*, double*, double*, double*, double*, double*, double*) = 0;
+  virtual int set_state(const int8_t*, const double*, const double*, const double*,
+                        const double*) = 0;
+  virtual void get_edge(int8_t*) = 0;
+  virtual int buckets(int which, int* cs, int* ids, int* count) = 0;
+  virtual int mirrors(int* ids, double* pos, double* prs, int* count) = 0;
+};
+
+template <class real>
+struct Sim : Base {
+  int dim = 3, nP = 0;
+  mps_params prm;
+  std::vector<signed char> Typ, WLLSE, TypM;
+  std::vector<real> Pos[3], Vel[3], Acc[3], WLLVec[3], PosM[3], VelM[3];
+  std::vector<real> Prs, pav, nd, PrsM;
+  std::vector<int> FromWLL;
+  real MINc[3], MAXc[3], G[3];
+  real PCL_DST, r, r2, rp, rp2, rlim, rlim2, rlim_wall, rlim_wall2, DB, DBinv, COL;
+  real n0, lmd, n0_grad, lmd_grad, umax, SND, Prs_coef, Pmax, dt, KNM_VSC, Dns;
+  real max_height, min_height, inset;
+  int Numn0_grad = 0, nB[3] = {1, 1, 1}, nBxy = 1, nBxyz = 1;
+  std::vector<int> bfst, blst, nxt, bfstM, blstM, nxtM;
+
+  // ---- grid (AlcBkt, kernel.cu:307-331; keys kernel.cu:521-524) ----------
+  inline int cellc(real x, int a) const {
+    real t = (x - MINc[a]) * DBinv;
+    t = std::fmax(t, real(0));
+    t = std::fmin(t, real(nB[a]));
+    int c = (int)t + 1;
+    return c < 1 ? 1 : (c > nB[a] - 2 ? nB[a] - 2 : c);
+  }
+  inline void cell3(real x, real y, real z, int& ix, int& iy, int& iz) const {
+    ix = cellc(x, 0);
+    iy = cellc(y, 1);
+    iz = dim == 3 ? cellc(z, 2) : 0;
+  }
+  inline int key(int ix, int iy, int iz) const { return iz * nBxy + iy * nB[0] + ix; }
+
+  // Visit list entries of the 27 (3-D) or 9 (2-D) neighbour cells in the
+  // reference's order: z outermost, then y, then x (kernel.cu:1038-1040).
+  template <class F>
+  inline void stencil(const std::vector<int>& head, const std::vector<int>& next, int ix, int iy,
+                      int iz, F&& f) const {
+    const int zlo = dim == 3 ? iz - 1 : 0, zhi = dim == 3 ? iz + 1 : 0;
+    for (int jz = zlo; jz <= zhi; jz++)
+      for (int jy = iy - 1; jy <= iy + 1; jy++)
+        for (int jx = ix - 1; jx <= ix + 1; jx++) {
+          int j = head[key(jx, jy, jz)];
+          if (j == -1) continue;
+          for (;;) {
+            f(j);
+            j = next[j];
+            if (j == -1) break;
+          }
+        }
+  }
+
+  int init(const mps_params* p, const mps_scene* s) {
+    prm = *p;
+    dim = p->dim;
+    nP = s->n;
+    PCL_DST = real(p->l0);
+    for (int a = 0; a < 3; a++) {
+      // kernel.cu:26-31: range expanded by 3.1*PCL_DST
+      MINc[a] = real(p->range_min[a] - p->re_factor * double(PCL_DST));
+      MAXc[a] = real(p->range_max[a] + p->re_factor * double(PCL_DST));
+      G[a] = real(p->gravity[a]);
+    }
+    if (dim == 2) { MINc[2] = MAXc[2] = real(0); G[2] = real(0); }
+    Typ.assign(nP, FLD);
+    WLLSE.assign(nP, Surface);
+    for (int a = 0; a < 3; a++) {
+      Pos[a].assign(nP, real(0));
+      Vel[a].assign(nP, real(0));
+      Acc[a].assign(nP, real(0));
+      WLLVec[a].assign(nP, real(0));
+    }
+    Prs.assign(nP, real(0));
+    pav.assign(nP, real(0));
+    nd.assign(nP, real(0));
+    for (int i = 0; i < nP; i++) {
+      Typ[i] = s->typ[i];
+      for (int a = 0; a < dim; a++) {
+        Pos[a][i] = real(s->pos[a * (size_t)nP + i]);
+        if (s->vel) Vel[a][i] = real(s->vel[a * (size_t)nP + i]);
+        if (is_wall(Typ[i]) && s->normal) WLLVec[a][i] = real(s->normal[a * (size_t)nP + i]);
+      }
+      if (s->prs) Prs[i] = real(s->prs[i]);
+      if (s->pav) pav[i] = real(s->pav[i]);
+    }
+    alc_bkt();
+    for (int i = 0; i < nP; i++) {
+      if (Typ[i] == GST) continue;
+      for (int a = 0; a < dim; a++)
+        if (!(Pos[a][i] > MINc[a] && Pos[a][i] < MAXc[a]))
+          return fail(MPS_E_ARG, "particle " + std::to_string(i) + " outside the domain");
+    }
+    set_para();
+    if (s->edge) {
+      for (int i = 0; i < nP; i++) WLLSE[i] = s->edge[i];
+    } else {
+      surface_edge(s);
+    }
+    TypM.assign((size_t)nP * NumMRR, GST);
+    FromWLL.assign((size_t)nP * NumMRR, -1);
+    for (int a = 0; a < 3; a++) {
+      PosM[a].assign((size_t)nP * NumMRR, real(0));
+      VelM[a].assign((size_t)nP * NumMRR, real(0));
+    }
+    PrsM.assign((size_t)nP * NumMRR, real(0));
+    bfstM.assign(nBxyz, -1);
+    blstM.assign(nBxyz, -1);
+    nxtM.assign((size_t)nP * NumMRR, -1);
+    bfst.assign(nBxyz, -1);
+    blst.assign(nBxyz, -1);
+    nxt.assign(nP, -1);
+    mk_bkt();  // ClcMPS starts with MkBkt (kernel.cu:1878)
+    return MPS_OK;
+  }
+
+  // kernel.cu:307-331
+  void alc_bkt() {
+    r = PCL_DST * real(prm.re_factor);
+    r2 = r * r;
+    rp = PCL_DST * real(prm.rp_factor);
+    rp2 = rp * rp;
+    DB = PCL_DST * real(prm.re_factor);
+    DBinv = real(1) / DB;
+    for (int a = 0; a < 3; a++) nB[a] = a < dim ? (int)((MAXc[a] - MINc[a]) * DBinv) + 3 : 1;
+    nBxy = nB[0] * nB[1];
+    nBxyz = nBxy * nB[2];
+  }
+
+  // kernel.cu:334-417
+  void set_para() {
+    real tn0 = 0, tn0_grad = 0, tlmd = 0, tlmd_grad = 0;
+    int tNum = 0;
+    const int zlo = dim == 3 ? -10 : 0, zhi = dim == 3 ? 10 : 1;
+    for (int ix = -10; ix < 10; ix++)
+      for (int iy = -10; iy < 10; iy++)
+        for (int iz = zlo; iz < zhi; iz++) {
+          real x = PCL_DST * real(ix);
+          real y = PCL_DST * real(iy);
+          real z = PCL_DST * real(iz);
+          real dist2 = x * x + y * y + z * z;
+          if (dist2 == real(0)) continue;
+          real dist = std::sqrt(dist2);
+          if (dist2 < rp2) {
+            tn0_grad += WEI_grad(dist, rp);
+            tlmd_grad += dist2 * WEI_grad(dist, rp);
+            tNum += 1;
+          }
+          if (dist2 < r2) {
+            tn0 += WEI(dist, r);
+            tlmd += dist2 * WEI(dist, r);
+          }
+        }
+    n0 = tn0;
+    lmd = tlmd / tn0;
+    n0_grad = tn0_grad;
+    lmd_grad = tlmd_grad / tn0_grad;
+    Numn0_grad = tNum;
+
+    rlim = PCL_DST * real(prm.dst_lmt_rat);
+    rlim2 = rlim * rlim;
+    COL = real(prm.restitution) + real(1);
+    Dns = real(prm.rho);
+    KNM_VSC = real(prm.nu);
+    rlim_wall = real(prm.rlim_wall_factor) * PCL_DST;  // kernel.cu:1184
+    rlim_wall2 = rlim_wall * rlim_wall;
+    inset = real(prm.re_factor) * PCL_DST;             // kernel.cu:484-486
+
+    // kernel.cu:399-406, including its if / else-if update
+    max_height = MINc[1];
+    min_height = MAXc[1];
+    for (int i = 0; i < nP; i++) {
+      if (Typ[i] == FLD) {
+        if (max_height < Pos[1][i]) max_height = Pos[1][i];
+        else if (min_height > Pos[1][i]) min_height = Pos[1][i];
+      }
+    }
+    umax = std::sqrt(real(2) * std::fabs(G[1]) * (max_height - min_height));
+    SND = prm.c > 0 ? real(prm.c) : umax / real(prm.mach);
+    Prs_coef = SND * SND / n0_grad;
+    if (prm.pmax_mode == MPS_PMAX_ABS_G)
+      Pmax = Dns * std::fabs(G[1]) * (max_height - min_height);
+    else if (prm.pmax_mode == MPS_PMAX_REFERENCE)
+      Pmax = Dns * G[1] * (max_height - min_height);
+    else
+      Pmax = real(INFINITY);
+    dt = prm.dt > 0 ? real(prm.dt) : real(PCL_DST / (umax + SND));
+  }
+
+  // d_Surface_Edge (kernel.cu:553-668), evaluated once in FP64 on the input
+  // positions with radius edge_factor*l0; normals compared in `real`.
+  void surface_edge(const mps_scene* s) {
+    const double R = prm.edge_factor * prm.l0, R2 = R * R;
+    std::unordered_map<long long, std::vector<int>> grid;
+    auto ck = [&](int i, int a) { return (long long)std::floor(s->pos[a * (size_t)nP + i] / R); };
+    auto hk = [](long long x, long long y, long long z) {
+      return ((x + (1 << 20)) << 42) ^ ((y + (1 << 20)) << 21) ^ (z + (1 << 20));
+    };
+    for (int i = 0; i < nP; i++)
+      if (is_wall(Typ[i])) grid[hk(ck(i, 0), ck(i, 1), dim == 3 ? ck(i, 2) : 0)].push_back(i);
+    for (int i = 0; i < nP; i++) {
+      if (!is_wall(Typ[i])) continue;
+      long long cx = ck(i, 0), cy = ck(i, 1), cz = dim == 3 ? ck(i, 2) : 0;
+      for (long long dz = (dim == 3 ? -1 : 0); dz <= (dim == 3 ? 1 : 0); dz++)
+        for (long long dy = -1; dy <= 1; dy++)
+          for (long long dx = -1; dx <= 1; dx++) {
+            auto it = grid.find(hk(cx + dx, cy + dy, cz + dz));
+            if (it == grid.end()) continue;
+            for (int j : it->second) {
+              if (j == i || Typ[j] != Typ[i]) continue;
+              double px = s->pos[i] - s->pos[j];
+              double py = s->pos[(size_t)nP + i] - s->pos[(size_t)nP + j];
+              double pz = dim == 3 ? s->pos[2 * (size_t)nP + i] - s->pos[2 * (size_t)nP + j] : 0.0;
+              double d2 = px * px + py * py + pz * pz;
+              if (d2 < R2) {
+                if (WLLVec[0][j] != WLLVec[0][i] || WLLVec[1][j] != WLLVec[1][i] ||
+                    WLLVec[2][j] != WLLVec[2][i])
+                  WLLSE[i] = Edge;
+              }
+            }
+          }
+    }
+  }
+
+  // ---- MkBkt (kernel.cu:516-550) with a defined in-cell order ------------
+  void mk_bkt() {
+    std::fill(bfst.begin(), bfst.end(), -1);
+    std::fill(blst.begin(), blst.end(), -1);
+    std::fill(nxt.begin(), nxt.end(), -1);
+    for (int pass = 0; pass < 2; pass++) {
+      for (int i = 0; i < nP; i++) {
+        if (Typ[i] == GST) continue;
+        if ((pass == 0) != (Typ[i] == FLD)) continue;
+        int ix, iy, iz;
+        cell3(Pos[0][i], Pos[1][i], dim == 3 ? Pos[2][i] : real(0), ix, iy, iz);
+        int ib = key(ix, iy, iz);
+        int j = blst[ib];
+        blst[ib] = i;
+        if (j == -1) bfst[ib] = i;
+        else nxt[j] = i;
+      }
+    }
+  }
+
+  // ---- ResetMRR (kernel.cu:691-701) --------------------------------------
+  void reset_mrr() {
+    const long long n = (long long)nP * NumMRR;
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < n; i++) {
+      TypM[i] = GST;
+      PosM[0][i] = PosM[1][i] = PosM[2][i] = real(0);
+      VelM[0][i] = VelM[1][i] = VelM[2][i] = real(0);
+      PrsM[i] = real(0);
+      FromWLL[i] = -1;
+    }
+  }
+
+  // ---- GenMRR_nonslip (kernel.cu:720-893) ----------------------------------
+  void gen_mrr() {
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int i = 0; i < nP; i++) {
+      if (Typ[i] != FLD) continue;
+      const real r2_2 = real(1) * r2;
+      const real px = Pos[0][i], py = Pos[1][i], pz = dim == 3 ? Pos[2][i] : real(0);
+      int WLLexist = 0, Edgeexist = 0;
+      const long long iNM = (long long)i * NumMRR;
+      int Edge_unique[NumMRR];
+      for (int k = 0; k < NumMRR; k++) Edge_unique[k] = -1;
+      int ix, iy, iz;
+      cell3(px, py, pz, ix, iy, iz);
+      stencil(bfst, nxt, ix, iy, iz, [&](int j) {
+        if (j == i) return;
+        if (!is_wall(Typ[j])) return;
+        real qx = px - Pos[0][j], qy = py - Pos[1][j];
+        real qz = dim == 3 ? pz - Pos[2][j] : real(0);
+        real dist2 = dim == 3 ? qx * qx + qy * qy + qz * qz : qx * qx + qy * qy;
+        if (dist2 <= r2_2) {
+          WLLexist = 1;
+          const real wx = WLLVec[0][j], wy = WLLVec[1][j], wz = WLLVec[2][j];
+          int unique = 0;
+          for (int k = 0; k < NumMRR; k++) {
+            int jNum = FromWLL[k + iNM];
+            if (jNum != -1 && WLLVec[0][jNum] == wx && WLLVec[1][jNum] == wy &&
+                WLLVec[2][jNum] == wz)
+              unique = 1;
+          }
+          if (unique == 0) {
+            for (int k = 0; k < NumMRR; k++)
+              if (FromWLL[k + iNM] == -1) { FromWLL[k + iNM] = j; break; }
+          }
+          if (WLLSE[j] == Edge) {
+            Edgeexist = 1;
+            int E_unique = 0;
+            for (int k = 0; k < NumMRR; k++) {
+              int q = Edge_unique[k];
+              if (q != -1 && WLLVec[0][q] == wx && WLLVec[1][q] == wy && WLLVec[2][q] == wz)
+                E_unique = 1;
+            }
+            if (E_unique == 0) {
+              for (int k = 0; k < NumMRR; k++)
+                if (Edge_unique[k] == -1) { Edge_unique[k] = j; break; }
+            }
+          }
+        }
+      });
+      if (WLLexist != 1) continue;
+      for (int k = 0; k < NumMRR; k++) {  // plane mirrors, kernel.cu:811-838
+        const long long kiNM = k + iNM;
+        const int F = FromWLL[kiNM];
+        if (F == -1) continue;
+        real wpx = Pos[0][F], wpy = Pos[1][F], wpz = dim == 3 ? Pos[2][F] : real(0);
+        real nx = WLLVec[0][F], ny = WLLVec[1][F], nz = WLLVec[2][F];
+        real PWx = px - wpx, PWy = py - wpy, PWz = pz - wpz;
+        real PW_WLL = dim == 3 ? PWx * nx + PWy * ny + PWz * nz : PWx * nx + PWy * ny;
+        if (PW_WLL < 0) continue;
+        real nn = dim == 3 ? nx * nx + ny * ny + nz * nz : nx * nx + ny * ny;
+        real distance = PW_WLL / std::sqrt(nn);
+        TypM[kiNM] = MRR;
+        PosM[0][kiNM] = wpx + PWx - real(2) * distance * nx;
+        PosM[1][kiNM] = wpy + PWy - real(2) * distance * ny;
+        if (dim == 3) PosM[2][kiNM] = wpz + PWz - real(2) * distance * nz;
+        PrsM[kiNM] = Prs[i];
+        VelM[0][kiNM] = VelM[1][kiNM] = VelM[2][kiNM] = real(0);
+      }
+      if (Edgeexist == 1) {  // edge mirror, kernel.cu:840-888
+        int NumEdge = 0;
+        for (int k = 0; k < NumMRR; k++)
+          if (Edge_unique[k] != -1) NumEdge++;
+        if (NumEdge >= 2) {
+          real wpx = 0, wpy = 0, wpz = 0, sx = 0, sy = 0, sz = 0, PWx = 0, PWy = 0, PWz = 0;
+          for (int k = 0; k < NumMRR; k++) {
+            const int F = Edge_unique[k];
+            if (F == -1) continue;
+            wpx = Pos[0][F];
+            wpy = Pos[1][F];
+            wpz = dim == 3 ? Pos[2][F] : real(0);
+            sx += WLLVec[0][F];
+            sy += WLLVec[1][F];
+            sz += WLLVec[2][F];
+            PWx = px - wpx;
+            PWy = py - wpy;
+            PWz = pz - wpz;
+          }
+          real abs_s = std::sqrt(dim == 3 ? sx * sx + sy * sy + sz * sz : sx * sx + sy * sy);
+          sx /= abs_s;
+          sy /= abs_s;
+          sz /= abs_s;
+          real PW_WLL = dim == 3 ? PWx * sx + PWy * sy + PWz * sz : PWx * sx + PWy * sy;
+          if (PW_WLL < 0) continue;
+          real distance =
+              PW_WLL / std::sqrt(dim == 3 ? sx * sx + sy * sy + sz * sz : sx * sx + sy * sy);
+          long long space = -1;
+          for (int k = 0; k < NumMRR; k++)
+            if (FromWLL[k + iNM] == -1) { space = k + iNM; break; }
+          if (space < 0) continue;
+          TypM[space] = MRR;
+          PosM[0][space] = wpx + PWx - real(2) * distance * sx;
+          PosM[1][space] = wpy + PWy - real(2) * distance * sy;
+          if (dim == 3) PosM[2][space] = wpz + PWz - real(2) * distance * sz;
+          VelM[0][space] = VelM[1][space] = VelM[2][space] = real(0);
+          PrsM[space] = Prs[i];
+        }
+      }
+    }
+  }
+
+  // ---- d_MRRinout (kernel.cu:932-985) --------------------------------------
+  void mrr_inout() {
+    const long long n = (long long)nP * NumMRR;
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (long long m = 0; m < n; m++) {
+      if (TypM[m] == GST) continue;
+      const real mx = PosM[0][m], my = PosM[1][m], mz = dim == 3 ? PosM[2][m] : real(0);
+      int WLLexist = 0;
+      real walldist = real(10) * r;
+      real wpx = 0, wpy = 0, wpz = 0, wx = 0, wy = 0, wz = 0;
+      int ix, iy, iz;
+      cell3(mx, my, mz, ix, iy, iz);
+      stencil(bfst, nxt, ix, iy, iz, [&](int j) {
+        if (!is_wall(Typ[j])) return;
+        real jx = Pos[0][j], jy = Pos[1][j], jz = dim == 3 ? Pos[2][j] : real(0);
+        real qx = mx - jx, qy = my - jy, qz = mz - jz;
+        real dist2 = dim == 3 ? qx * qx + qy * qy + qz * qz : qx * qx + qy * qy;
+        real dist = std::sqrt(dist2);
+        if (dist <= r && dist < walldist) {
+          WLLexist = 1;
+          walldist = dist;
+          wx = WLLVec[0][j];
+          wy = WLLVec[1][j];
+          wz = WLLVec[2][j];
+          wpx = jx;
+          wpy = jy;
+          wpz = jz;
+        }
+      });
+      real inout = dim == 3 ? (mx - wpx) * wx + (my - wpy) * wy + (mz - wpz) * wz
+                            : (mx - wpx) * wx + (my - wpy) * wy;
+      if (WLLexist == 1 && inout >= 0) TypM[m] = GST;
+    }
+  }
+
+  // ---- d_MkBkt_MRR (kernel.cu:916-929), slots in ascending order ----------
+  void mk_bkt_mrr() {
+    std::fill(bfstM.begin(), bfstM.end(), -1);
+    std::fill(blstM.begin(), blstM.end(), -1);
+    std::fill(nxtM.begin(), nxtM.end(), -1);
+    const long long n = (long long)nP * NumMRR;
+    for (long long m = 0; m < n; m++) {
+      if (TypM[m] == GST) continue;
+      int ix, iy, iz;
+      cell3(PosM[0][m], PosM[1][m], dim == 3 ? PosM[2][m] : real(0), ix, iy, iz);
+      int ib = key(ix, iy, iz);
+      int j = blstM[ib];
+      blstM[ib] = (int)m;
+      if (j == -1) bfstM[ib] = (int)m;
+      else nxtM[j] = (int)m;
+    }
+  }
+
+  void mirror_phase() {  // kernel.cu:1906-1908 / 1925-1927
+    reset_mrr();
+    gen_mrr();
+    mrr_inout();
+    mk_bkt_mrr();
+  }
+
+  // ---- d_VscTrm (kernel.cu:1013-1119) --------------------------------------
+  void vsc_trm() {
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int i = 0; i < nP; i++) {
+      if (Typ[i] != FLD) continue;
+      const real px = Pos[0][i], py = Pos[1][i], pz = dim == 3 ? Pos[2][i] : real(0);
+      const real vx = Vel[0][i], vy = Vel[1][i], vz = Vel[2][i];
+      real ax = 0, ay = 0, az = 0;
+      const real invn0 = real(1) / n0;
+      int ix, iy, iz;
+      cell3(px, py, pz, ix, iy, iz);
+      stencil(bfst, nxt, ix, iy, iz, [&](int j) {
+        if (j == i || Typ[j] != FLD) return;
+        real qx = Pos[0][j] - px, qy = Pos[1][j] - py;
+        real qz = dim == 3 ? Pos[2][j] - pz : real(0);
+        real dist2 = dim == 3 ? qx * qx + qy * qy + qz * qz : qx * qx + qy * qy;
+        real dist = std::sqrt(dist2);
+        if (dist <= r) {
+          ax += real(2) * (Vel[0][j] - vx) * PCL_DST / dist / dist / dist;
+          ay += real(2) * (Vel[1][j] - vy) * PCL_DST / dist / dist / dist;
+          if (dim == 3) az += real(2) * (Vel[2][j] - vz) * PCL_DST / dist / dist / dist;
+        }
+      });
+      stencil(bfstM, nxtM, ix, iy, iz, [&](int j) {
+        real qx = PosM[0][j] - px, qy = PosM[1][j] - py;
+        real qz = dim == 3 ? PosM[2][j] - pz : real(0);
+        real dist2 = dim == 3 ? qx * qx + qy * qy + qz * qz : qx * qx + qy * qy;
+        real dist = std::sqrt(dist2);
+        if (dist <= r) {
+          ax += real(2) * (VelM[0][j] - vx) * PCL_DST / dist / dist / dist;
+          ay += real(2) * (VelM[1][j] - vy) * PCL_DST / dist / dist / dist;
+          if (dim == 3) az += real(2) * (VelM[2][j] - vz) * PCL_DST / dist / dist / dist;
+        }
+      });
+      const real n0KNM = invn0 * KNM_VSC;
+      Acc[0][i] = n0KNM * ax + G[0];
+      Acc[1][i] = n0KNM * ay + G[1];
+      if (dim == 3) Acc[2][i] = n0KNM * az + G[2];
+    }
+  }
+
+  // ---- d_ChkPcl (kernel.cu:481-513) --------------------------------------
+  inline void chk_pcl(int i) {
+    if (Typ[i] != FLD) return;
+    bool in = Pos[0][i] < (MAXc[0] - inset) && Pos[0][i] > (MINc[0] + inset) &&
+              Pos[1][i] < (MAXc[1] - inset) && Pos[1][i] > (MINc[1] + inset);
+    if (dim == 3) in = in && Pos[2][i] < (MAXc[2] - inset) && Pos[2][i] > (MINc[2] + inset);
+    if (in) {
+      real ux = Vel[0][i], uy = Vel[1][i], uz = Vel[2][i];
+      real U = dim == 3 ? ux * ux + uy * uy + uz * uz : ux * ux + uy * uy;
+      U = std::sqrt(U);
+      if (U > umax) {
+        ux *= umax / U;
+        uy *= umax / U;
+        uz *= umax / U;
+        Vel[0][i] = ux;
+        Vel[1][i] = uy;
+        if (dim == 3) Vel[2][i] = uz;
+      }
+    } else {
+      Typ[i] = GST;
+      for (int a = 0; a < 3; a++) Pos[a][i] = Vel[a][i] = Acc[a][i] = real(0);
+      Prs[i] = real(0);
+    }
+  }
+
+  // ---- d_UpPcl1 (kernel.cu:1138-1149) ----------------------------------------
+  void up_pcl1() {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < nP; i++) {
+      if (Typ[i] == FLD) {
+        for (int a = 0; a < dim; a++) Vel[a][i] += Acc[a][i] * dt;
+        for (int a = 0; a < dim; a++) Pos[a][i] += Vel[a][i] * dt;
+        Acc[0][i] = Acc[1][i] = Acc[2][i] = real(0);
+      }
+      chk_pcl(i);
+    }
+  }
+
+  // ---- d_ChkCol + d_UpChkCol (kernel.cu:1168-1287) ---------------------------
+  void chk_col() {
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int i = 0; i < nP; i++) {
+      if (Typ[i] != FLD) continue;
+      const real px = Pos[0][i], py = Pos[1][i], pz = dim == 3 ? Pos[2][i] : real(0);
+      const real vx = Vel[0][i], vy = Vel[1][i], vz = dim == 3 ? Vel[2][i] : real(0);
+      real v2x = vx, v2y = vy, v2z = vz;
+      const real mi = Dns;
+      int ix, iy, iz;
+      cell3(px, py, pz, ix, iy, iz);
+      stencil(bfst, nxt, ix, iy, iz, [&](int j) {
+        if (j == i) return;
+        const int typ = Typ[j];
+        real lim2;
+        if (typ == FLD) lim2 = rlim2;
+        else if (is_wall(typ)) lim2 = rlim_wall2;
+        else return;
+        real qx = Pos[0][j] - px, qy = Pos[1][j] - py;
+        real qz = dim == 3 ? Pos[2][j] - pz : real(0);
+        real dist2 = dim == 3 ? qx * qx + qy * qy + qz * qz : qx * qx + qy * qy;
+        if (dist2 < lim2) {
+          real fDT = dim == 3 ? (vx - Vel[0][j]) * qx + (vy - Vel[1][j]) * qy + (vz - Vel[2][j]) * qz
+                              : (vx - Vel[0][j]) * qx + (vy - Vel[1][j]) * qy;
+          if (fDT > real(0)) {
+            const real mj = Dns;
+            fDT *= COL * mj / (mi + mj) / dist2;
+            v2x -= qx * fDT;
+            v2y -= qy * fDT;
+            if (dim == 3) v2z -= qz * fDT;
+          }
+        }
+      });
+      const real mj = Dns;
+      stencil(bfstM, nxtM, ix, iy, iz, [&](int j) {
+        real qx = PosM[0][j] - px, qy = PosM[1][j] - py;
+        real qz = dim == 3 ? PosM[2][j] - pz : real(0);
+        real dist2 = dim == 3 ? qx * qx + qy * qy + qz * qz : qx * qx + qy * qy;
+        if (dist2 < rlim2) {
+          real fDT = dim == 3
+                         ? (vx - VelM[0][j]) * qx + (vy - VelM[1][j]) * qy + (vz - VelM[2][j]) * qz
+                         : (vx - VelM[0][j]) * qx + (vy - VelM[1][j]) * qy;
+          if (fDT > real(0)) {
+            fDT *= COL * mj / (mi + mj) / dist2;
+            v2x -= qx * fDT;
+            v2y -= qy * fDT;
+            if (dim == 3) v2z -= qz * fDT;
+          }
+        }
+      });
+      Acc[0][i] = v2x;
+      Acc[1][i] = v2y;
+      if (dim == 3) Acc[2][i] = v2z;
+    }
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < nP; i++) {
+      if (Typ[i] != FLD) continue;
+      for (int a = 0; a < dim; a++) Vel[a][i] = Acc[a][i];
+    }
+  }
+
+  // ---- d_MkPrs (kernel.cu:1306-1411) ----------------------------------------
+  void mk_prs() {
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int i = 0; i < nP; i++) {
+      if (Typ[i] != FLD) continue;
+      const real px = Pos[0][i], py = Pos[1][i], pz = dim == 3 ? Pos[2][i] : real(0);
+      real ni = 0;
+      int ix, iy, iz;
+      cell3(px, py, pz, ix, iy, iz);
+      stencil(bfst, nxt, ix, iy, iz, [&](int j) {
+        if (j == i || Typ[j] != FLD) return;
+        real qx = Pos[0][j] - px, qy = Pos[1][j] - py;
+        real qz = dim == 3 ? Pos[2][j] - pz : real(0);
+        real dist2 = dim == 3 ? qx * qx + qy * qy + qz * qz : qx * qx + qy * qy;
+        real dist = std::sqrt(dist2);
+        if (dist < rp) ni += WEI_grad(dist, rp);
+      });
+      stencil(bfstM, nxtM, ix, iy, iz, [&](int j) {
+        real qx = PosM[0][j] - px, qy = PosM[1][j] - py;
+        real qz = dim == 3 ? PosM[2][j] - pz : real(0);
+        real dist2 = dim == 3 ? qx * qx + qy * qy + qz * qz : qx * qx + qy * qy;
+        real dist = std::sqrt(dist2);
+        if (dist < rp) ni += WEI_grad(dist, rp);
+      });
+      const real mi = Dns;
+      real pressure = real(ni > n0_grad) * (ni - n0_grad) * Prs_coef * mi;
+      if (pressure <= Pmax && pressure >= real(0)) {
+      } else {
+        if (pressure > Pmax) pressure = Pmax;
+        else pressure = real(0);
+      }
+      Prs[i] = pressure;
+      nd[i] = ni;
+    }
+  }
+
+  // ---- d_PrsGrdTrm, active version (kernel.cu:1651-1794) --------------------
+  void prs_grd() {
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int i = 0; i < nP; i++) {
+      if (Typ[i] != FLD) continue;
+      const real invro = real(-1) / Dns;
+      const real px = Pos[0][i], py = Pos[1][i], pz = dim == 3 ? Pos[2][i] : real(0);
+      real Prs_min = Prs[i];
+      real ax = 0, ay = 0, az = 0;
+      const real A = real(dim) / n0_grad;
+      int ix, iy, iz;
+      cell3(px, py, pz, ix, iy, iz);
+      stencil(bfst, nxt, ix, iy, iz, [&](int j) {
+        if (j == i || Typ[j] != FLD) return;
+        real qx = Pos[0][j] - px, qy = Pos[1][j] - py;
+        real qz = dim == 3 ? Pos[2][j] - pz : real(0);
+        real dist2 = dim == 3 ? qx * qx + qy * qy + qz * qz : qx * qx + qy * qy;
+        if (dist2 < rp2) {
+          real prs = Prs[j];
+          if (prs < Prs_min) Prs_min = prs;
+        }
+      });
+      stencil(bfst, nxt, ix, iy, iz, [&](int j) {
+        if (j == i || Typ[j] != FLD) return;
+        real qx = Pos[0][j] - px, qy = Pos[1][j] - py;
+        real qz = dim == 3 ? Pos[2][j] - pz : real(0);
+        real dist2 = dim == 3 ? qx * qx + qy * qy + qz * qz : qx * qx + qy * qy;
+        if (dist2 < rp2) {
+          real dist = std::sqrt(dist2);
+          real w = WEI_grad(dist, rp) / dist2;
+          real d = Prs[j] - Prs_min;
+          ax += d * w * qx;
+          ay += d * w * qy;
+          if (dim == 3) az += d * w * qz;
+        }
+      });
+      stencil(bfstM, nxtM, ix, iy, iz, [&](int j) {
+        real qx = PosM[0][j] - px, qy = PosM[1][j] - py;
+        real qz = dim == 3 ? PosM[2][j] - pz : real(0);
+        real dist2 = dim == 3 ? qx * qx + qy * qy + qz * qz : qx * qx + qy * qy;
+        if (dist2 < rp2) {
+          real dist = std::sqrt(dist2);
+          real w = WEI_grad(dist, rp) / dist2;
+          real d = PrsM[j] - Prs_min;
+          ax += d * w * qx;
+          ay += d * w * qy;
+          if (dim == 3) az += d * w * qz;
+        }
+      });
+      Acc[0][i] = invro * ax * A;
+      Acc[1][i] = invro * ay * A;
+      if (dim == 3) Acc[2][i] = invro * az * A;
+    }
+  }
+
+  // ---- d_UpPcl2 (kernel.cu:1815-1829) ----------------------------------------
+  void up_pcl2() {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < nP; i++) {
+      if (Typ[i] == FLD) {
+        real a[3] = {Acc[0][i], Acc[1][i], Acc[2][i]};
+        for (int k = 0; k < dim; k++) Vel[k][i] += a[k] * dt;
+        for (int k = 0; k < dim; k++) Pos[k][i] += a[k] * dt * dt;
+        Acc[0][i] = Acc[1][i] = Acc[2][i] = real(0);
+      }
+      pav[i] += Prs[i];
+      chk_pcl(i);
+    }
+  }
+
+  // ---- MPS::ClcMPS loop body (kernel.cu:1904-1935) ----------------------------
+  int step(int n) override {
+    for (int s = 0; s < n; s++) {
+      mk_bkt();
+      mirror_phase();
+      vsc_trm();
+      up_pcl1();
+      mk_bkt();
+      chk_col();
+      mk_bkt();
+      mk_prs();
+      mirror_phase();
+      prs_grd();
+      up_pcl2();
+    }
+    return MPS_OK;
+  }
+
+  int phase(int p) override {
+    switch (p) {
+      case MPS_PH_MKBKT: mk_bkt(); break;
+      case MPS_PH_MIRROR: mirror_phase(); break;
+      case MPS_PH_VSC: vsc_trm(); break;
+      case MPS_PH_UP1: up_pcl1(); break;
+      case MPS_PH_COL: chk_col(); break;
+      case MPS_PH_PRS: mk_prs(); break;
+      case MPS_PH_GRAD: prs_grd(); break;
+      case MPS_PH_UP2: up_pcl2(); break;
+      default: return fail(MPS_E_ARG, "unknown phase");
+    }
+    return MPS_OK;
+  }
+
+  void constants(mps_constants* c) override {
+    std::memset(c, 0, sizeof(*c));
+    c->n0 = n0;
+    c->lmd = lmd;
+    c->n0_grad = n0_grad;
+    c->lmd_grad = lmd_grad;
+    c->num_n0_grad = Numn0_grad;
+    for (int a = 0; a < 3; a++) {
+      c->nb[a] = nB[a];
+      c->min_c[a] = MINc[a];
+      c->max_c[a] = MAXc[a];
+    }
+    c->ncell = nBxyz;
+    int nf = 0, nw = 0;
+    for (int i = 0; i < nP; i++) (is_wall(Typ[i]) ? nw : nf)++;
+    c->n_fluid = nf;
+    c->n_wall = nw;
+    c->r = r;
+    c->rp = rp;
+    c->rlim = rlim;
+    c->rlim_wall = rlim_wall;
+    c->db = DB;
+    c->dbinv = DBinv;
+    c->umax = umax;
+    c->snd = SND;
+    c->prs_coef = Prs_coef;
+    c->pmax = Pmax;
+    c->dt = dt;
+    c->col = COL;
+    c->max_height = max_height;
+    c->min_height = min_height;
+  }
+
+  void get_state(int8_t* typ, double* pos, double* vel, double* acc, double* prs, double* pav_o,
+                 double* nd_o) override {
+    const size_t n = nP;
+    for (int i = 0; i < nP; i++) {
+      if (typ) typ[i] = Typ[i];
+      for (int a = 0; a < 3; a++) {
+        const bool z = (a == 2 && dim == 2);
+        if (pos) pos[a * n + i] = z ? 0.0 : double(Pos[a][i]);
+        if (vel) vel[a * n + i] = z ? 0.0 : double(Vel[a][i]);
+        if (acc) acc[a * n + i] = z ? 0.0 : double(Acc[a][i]);
+      }
+      if (prs) prs[i] = Prs[i];
+      if (pav_o) pav_o[i] = pav[i];
+      if (nd_o) nd_o[i] = nd[i];
+    }
+  }
+
+  int set_state(const int8_t* typ, const double* pos, const double* vel, const double* prs,
+                const double* pav_i) override {
+    const size_t n = nP;
+    for (int i = 0; i < nP; i++) {
+      if (typ) {
+        if (is_wall(Typ[i]) != is_wall(typ[i])) return fail(MPS_E_ARG, "wall set may not change");
+        Typ[i] = typ[i];
+      }
+      for (int a = 0; a < dim; a++) {
+        if (pos && !is_wall(Typ[i])) Pos[a][i] = real(pos[a * n + i]);
+        if (vel) Vel[a][i] = real(vel[a * n + i]);
+      }
+      if (prs) Prs[i] = real(prs[i]);
+      if (pav_i) pav[i] = real(pav_i[i]);
+    }
+    return MPS_OK;
+  }
+
+  void get_edge(int8_t* e) override {
+    for (int i = 0; i < nP; i++) e[i] = WLLSE[i];
+  }
+
+  int buckets(int which, int* cs, int* ids, int* count) override {
+    int k = 0;
+    const bool mir = which == MPS_BKT_MIRROR;
+    const std::vector<int>& head = mir ? bfstM : bfst;
+    const std::vector<int>& next = mir ? nxtM : nxt;
+    for (int c = 0; c < nBxyz; c++) {
+      if (cs) cs[c] = k;
+      for (int j = head[c]; j != -1; j = next[j]) {
+        bool take = mir || (which == MPS_BKT_FLUID ? Typ[j] == FLD : is_wall(Typ[j]));
+        if (!take) continue;
+        if (ids) ids[k] = j;
+        k++;
+      }
+    }
+    if (cs) cs[nBxyz] = k;
+    *count = k;
+    return MPS_OK;
+  }
+
+  int mirrors(int* ids, double* pos, double* prs, int* count) override {
+    int m = 0;
+    buckets(MPS_BKT_MIRROR, nullptr, nullptr, &m);
+    if (ids || pos || prs) {
+      int k = 0;
+      for (int c = 0; c < nBxyz; c++)
+        for (int j = bfstM[c]; j != -1; j = nxtM[j]) {
+          if (ids) ids[k] = j;
+          if (pos) {
+            pos[k] = PosM[0][j];
+            pos[(size_t)m + k] = PosM[1][j];
+            pos[2 * (size_t)m + k] = dim == 3 ? PosM[2][j] : 0.0;
+          }
+          if (prs) prs[k] = PrsM[j];
+          k++;
+        }
+    }
+    *count = m;
+    return MPS_OK;
+  }
+};
+
+}  // namespace
+
+struct orc_handle {
+  Base* sim;
+};
+
+extern "C" {
+
+int orc_create(const mps_params* p, const mps_scene* s, orc_handle** out) {
+  if (!p || !s || !out) return fail(MPS_E_ARG, "null argument");
+  if (p->dim != 2 && p->dim != 3) return fail(MPS_E_ARG, "dim must be 2 or 3");
+  if (s->n <= 0 || !s->typ || !s->pos) return fail(MPS_E_ARG, "empty scene");
+  for (int i = 0; i < s->n; i++) {
+    int t = s->typ[i];
+    if (t != GST && t != FLD && !is_wall(t)) return fail(MPS_E_ARG, "bad particle type");
+  }
+  Base* b = nullptr;
+  int rc;
+  if (p->precision == MPS_FP64) {
+    auto* x = new Sim<double>();
+    rc = x->init(p, s);
+    b = x;
+  } else {
+    auto* x = new Sim<float>();
+    rc = x->init(p, s);
+    b = x;
+  }
+  if (rc != MPS_OK) {
+    delete b;
+    return rc;
+  }
+  *out = new orc_handle{b};
+  return MPS_OK;
+}
+
+int orc_destroy(orc_handle* h) {
+  if (h) {
+    delete h->sim;
+    delete h;
+  }
+  return MPS_OK;
+}
+
+int orc_set_threads(int n) {
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+}
+
+int orc_step(orc_handle* h, int n) { return h->sim->step(n); }
+int orc_phase(orc_handle* h, int p) { return h->sim->phase(p); }
+int orc_get_constants(orc_handle* h, mps_constants* c) {
+  h->sim->constants(c);
+  return MPS_OK;
+}
+int orc_get_state(orc_handle* h, int8_t* typ, double* pos, double* vel, double* acc, double* prs,
+                  double* pav, double* nd) {
+  h->sim->get_state(typ, pos, vel, acc, prs, pav, nd);
+  return MPS_OK;
+}
+int orc_set_state(orc_handle* h, const int8_t* typ, const double* pos, const double* vel,
+                  const double* prs, const double* pav) {
+  return h->sim->set_state(typ, pos, vel, prs, pav);
+}
+int orc_get_edge(orc_handle* h, int8_t* e) {
+  h->sim->get_edge(e);
+  return MPS_OK;
+}
+int orc_get_buckets(orc_handle* h, int which, int* cs, int* ids, int* count) {
+  return h->sim->buckets(which, cs, ids, count);
+}
+int orc_get_mirrors(orc_handle* h, int* ids, double* pos, double* prs, int* count) {
+  return h->sim->mirrors(ids, pos, prs, count);
+}
+const char* orc_last_error(void) { return g_err.c_str(); }
+
+}  // extern "C"

@@ -1,0 +1,52 @@
+"""Comparison helpers for the HIP path against the CPU oracle."""
+from __future__ import annotations
+
+import numpy as np
+
+FIELDS = ("typ", "pos", "vel", "acc", "prs", "pav", "nd")
+
+
+def first_diff(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    if a.shape != b.shape:
+        return f"shape {a.shape} vs {b.shape}"
+    bad = np.argwhere(~((a == b) | (np.isnan(a) & np.isnan(b)) if a.dtype.kind == "f" else (a == b)))
+    if len(bad) == 0:
+        return None
+    idx = tuple(bad[0])
+    return f"{len(bad)} differ; first at {idx}: {a[idx]!r} vs {b[idx]!r}"
+
+
+def assert_states_equal(gpu: dict, ref: dict, where: str, fields=FIELDS):
+    """Bit-exact equality of every state field (NaN == NaN)."""
+    for f in fields:
+        d = first_diff(gpu[f], ref[f])
+        assert d is None, f"{where}: field {f}: {d}"
+
+
+def assert_buckets_equal(h_gpu, h_ref, which, where):
+    cs_g, ids_g = h_gpu.buckets(which)
+    cs_r, ids_r = h_ref.buckets(which)
+    assert np.array_equal(cs_g, cs_r), f"{where}: cell_start differs: {first_diff(cs_g, cs_r)}"
+    assert np.array_equal(ids_g, ids_r), f"{where}: ids differ: {first_diff(ids_g, ids_r)}"
+
+
+def assert_mirrors_equal(h_gpu, h_ref, where):
+    ig, pg, qg = h_gpu.mirrors()
+    ir, pr, qr = h_ref.mirrors()
+    assert np.array_equal(ig, ir), f"{where}: mirror ids differ ({len(ig)} vs {len(ir)}): {first_diff(ig, ir)}"
+    assert first_diff(pg, pr) is None, f"{where}: mirror pos: {first_diff(pg, pr)}"
+    assert first_diff(qg, qr) is None, f"{where}: mirror prs: {first_diff(qg, qr)}"
+
+
+def rel_inf(a, b, mask=None):
+    """Normwise relative difference max|a-b| / max|b| over `mask`."""
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    if mask is not None:
+        a = a[..., mask]
+        b = b[..., mask]
+    den = np.max(np.abs(b)) if b.size else 0.0
+    num = np.max(np.abs(a - b)) if b.size else 0.0
+    return 0.0 if num == 0 else num / (den if den > 0 else 1.0)

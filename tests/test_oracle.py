@@ -1,0 +1,429 @@
+"""Pins the CPU oracle (oracle/mps_oracle.cpp) without the reference binary.
+
+The reference ships no tests, fixtures or golden vectors, so every check here
+comes from closed forms, invariants, brute force on tiny inputs, or the
+published lattice constants.  Each is chosen so that a plausible slip (a
+dropped term, a sign, a swapped index, a wrong radius test) fails it.
+"""
+import itertools
+import math
+
+import numpy as np
+import pytest
+
+from mps_cuda_amd import configs, ffi
+from tests.parity import assert_states_equal, rel_inf
+
+L0 = 0.01
+
+
+def scene_of(dim, fluid, walls=(), normals=(), vel=None, extent=1.0):
+    """Scene with the given fluid and wall positions (rows = particles)."""
+    fluid = np.asarray(fluid, float).reshape(-1, 3)
+    walls = np.asarray(walls, float).reshape(-1, 3)
+    normals = np.asarray(normals, float).reshape(-1, 3)
+    n = len(fluid) + len(walls)
+    typ = np.array([ffi.FLD] * len(fluid) + [ffi.WLL] * len(walls), np.int8)
+    pos = np.concatenate([fluid, walls]).T.copy()
+    nrm = np.zeros((3, n))
+    if len(walls):
+        nrm[:, len(fluid):] = normals.T
+    sc = ffi.SceneArrays(typ=typ, pos=pos, normal=nrm,
+                         range_min=(-extent, -extent, -extent if dim == 3 else 0.0),
+                         range_max=(extent, extent, extent if dim == 3 else 0.0))
+    if vel is not None:
+        sc.vel = np.asarray(vel, float).reshape(-1, 3).T.copy()
+        if sc.vel.shape[1] < n:
+            sc.vel = np.concatenate([sc.vel, np.zeros((3, n - sc.vel.shape[1]))], axis=1)
+    return sc
+
+
+def params_of(dim, precision="fp64", **kw):
+    base = dict(configs.COMMON, dt=1e-4)
+    base.update(kw)
+    return ffi.default_params(dim=dim, precision=precision, l0=L0,
+                              range_min=(-1.0, -1.0, -1.0 if dim == 3 else 0.0),
+                              range_max=(1.0, 1.0, 1.0 if dim == 3 else 0.0), **base)
+
+
+def lattice(dim, half, spacing=L0, center=(0.0, 0.0, 0.0)):
+    r = range(-half, half + 1)
+    pts = []
+    for idx in itertools.product(r, repeat=dim):
+        p = [center[a] + spacing * idx[a] for a in range(dim)] + [0.0] * (3 - dim)
+        pts.append(p)
+    return np.array(pts)
+
+
+def idx_of(pts, target):
+    return int(np.argmin(np.sum((pts - np.asarray(target)) ** 2, axis=1)))
+
+
+# --- SetPara lattice sums (kernel.cu:336-366) -----------------------------------
+
+PUBLISHED = {  # n0, n0_grad, lambda/l0^2, count within rp
+    3: (24.3281384117, 23.1676376897, 2.43276574985, 32),
+    2: (9.96322520228, 10.1316711289, 1.86822875131, 12),
+}
+
+
+def shell_sums(dim, re, rp):
+    """Same sums by shells: count integer vectors per |v|^2, then weight."""
+    count = {}
+    for v in itertools.product(range(-4, 5), repeat=dim):
+        k = sum(x * x for x in v)
+        if k:
+            count[k] = count.get(k, 0) + 1
+    n0 = sum(c * (re / math.sqrt(k) + math.sqrt(k) / re - 2) for k, c in count.items() if k < re * re)
+    lm = sum(c * k * (re / math.sqrt(k) + math.sqrt(k) / re - 2) for k, c in count.items() if k < re * re) / n0
+    n0g = sum(c * (rp / math.sqrt(k) - math.sqrt(k) / rp) for k, c in count.items() if k < rp * rp)
+    ng = sum(c for k, c in count.items() if k < rp * rp)
+    return n0, n0g, lm, ng
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_lattice_constants(oracle_mod, dim):
+    o = oracle_mod.Oracle(params_of(dim), scene_of(dim, [[0, 0, 0]]))
+    c = o.constants()
+    n0, n0g, lam, ng = PUBLISHED[dim]
+    s_n0, s_n0g, s_lam, s_ng = shell_sums(dim, 3.1, 2.1)
+    assert c["num_n0_grad"] == ng == s_ng
+    for got, pub, sh in ((c["n0"], n0, s_n0), (c["n0_grad"], n0g, s_n0g), (c["lmd"] / L0**2, lam, s_lam)):
+        assert abs(got - pub) < 1e-10 * pub
+        assert abs(got - sh) < 1e-12 * sh
+
+
+# --- buckets (MkBkt, kernel.cu:516-529) -------------------------------------------
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_buckets_match_brute_force(oracle_mod, dim):
+    rng = np.random.default_rng(dim)
+    n = 400 if dim == 3 else 300
+    pts = np.zeros((n, 3))
+    pts[:, :dim] = rng.uniform(-0.04, 0.04, (n, dim))
+    p = params_of(dim)
+    o = oracle_mod.Oracle(p, scene_of(dim, pts))
+    o.phase(ffi.PH_MKBKT)
+    c = o.constants()
+    cs, ids = o.buckets(ffi.BKT_FLUID)
+    assert len(ids) == n and sorted(ids) == list(range(n))
+    nb = c["nb"]
+    cell_of = np.empty(n, int)
+    for k in range(c["ncell"]):
+        seg = ids[cs[k]:cs[k + 1]]
+        assert np.all(np.diff(seg) > 0), "ids inside a cell must ascend"
+        cell_of[seg] = k
+    # independent cell coordinates
+    coords = np.floor((pts[:, :dim] - np.array(c["min_c"][:dim])) / c["db"]).astype(int) + 1
+    key = coords[:, 0] + nb[0] * coords[:, 1] + (nb[0] * nb[1] * coords[:, 2] if dim == 3 else 0)
+    assert np.array_equal(key, cell_of)
+    # the 27 / 9 neighbour cells hold every pair within r
+    d2 = np.sum((pts[:, None, :] - pts[None, :, :]) ** 2, axis=2)
+    close = np.argwhere((d2 <= c["r"] ** 2) & ~np.eye(n, dtype=bool))
+    assert len(close) > 0
+    assert np.all(np.abs(coords[close[:, 0]] - coords[close[:, 1]]) <= 1)
+
+
+# --- VscTrm (kernel.cu:1013-1119) ------------------------------------------------
+
+def test_isolated_particle_feels_gravity_only(oracle_mod):
+    o = oracle_mod.Oracle(params_of(3), scene_of(3, [[0.1, 0.2, 0.3], [-0.3, 0.0, 0.0]],
+                                                 vel=[[0.5, -0.2, 0.1], [0, 0, 0]]))
+    o.phase(ffi.PH_MKBKT)
+    o.phase(ffi.PH_VSC)
+    acc = o.state()["acc"]
+    assert np.array_equal(acc[:, 0], [0.0, -9.81, 0.0])
+
+
+def test_uniform_velocity_has_no_viscous_term(oracle_mod):
+    pts = lattice(3, 3)
+    v = np.tile([0.3, -0.7, 0.2], (len(pts), 1))
+    o = oracle_mod.Oracle(params_of(3), scene_of(3, pts, vel=v))
+    o.phase(ffi.PH_MKBKT)
+    o.phase(ffi.PH_VSC)
+    acc = o.state()["acc"]
+    assert np.allclose(acc, np.array([0, -9.81, 0])[:, None], atol=1e-12)
+
+
+def test_viscous_pair_is_antisymmetric_with_inverse_cube_weight(oracle_mod):
+    nu = 1e-3
+    p = params_of(2, nu=nu, gravity=(0, 0, 0))
+    for d in (0.5 * L0, 1.3 * L0, 3.0 * L0):
+        o = oracle_mod.Oracle(p, scene_of(2, [[0, 0, 0], [d, 0, 0]], vel=[[0.1, 0.2, 0], [-0.3, 0.5, 0]]))
+        o.phase(ffi.PH_MKBKT)
+        o.phase(ffi.PH_VSC)
+        acc = o.state()["acc"]
+        n0 = o.constants()["n0"]
+        assert np.allclose(acc[:, 0], -acc[:, 1], rtol=0, atol=1e-15)  # momentum conserved
+        dv = np.array([-0.4, 0.3, 0.0])
+        assert np.allclose(acc[:, 0], nu / n0 * 2 * dv * L0 / d**3, rtol=1e-13)
+    o = oracle_mod.Oracle(p, scene_of(2, [[0, 0, 0], [3.2 * L0, 0, 0]], vel=[[0.1, 0, 0], [-0.3, 0, 0]]))
+    o.phase(ffi.PH_MKBKT)
+    o.phase(ffi.PH_VSC)
+    assert np.all(o.state()["acc"] == 0)  # beyond re = 3.1 l0
+
+
+# --- ChkCol (kernel.cu:1168-1287) -------------------------------------------------
+
+@pytest.mark.parametrize("e", [0.0, 0.2, 1.0])
+def test_head_on_collision_restitution(oracle_mod, e):
+    u, d = 0.3, 0.4 * L0
+    p = params_of(2, restitution=e, gravity=(0, 0, 0))
+    o = oracle_mod.Oracle(p, scene_of(2, [[0, 0, 0], [d, 0, 0]], vel=[[u, 0.1, 0], [-u, 0.1, 0]]))
+    o.phase(ffi.PH_MKBKT)
+    o.phase(ffi.PH_COL)
+    v = o.state()["vel"]
+    assert np.allclose(v[0, 0] - v[0, 1], -e * 2 * u, rtol=1e-13, atol=1e-16)  # normal part reversed, scaled by e
+    assert np.allclose(v[1], [0.1, 0.1], rtol=1e-15)  # tangential part kept
+    assert np.isclose(v[0, 0] + v[0, 1], 0.0, atol=1e-16)  # momentum
+
+
+def test_collision_only_when_approaching_and_within_radius(oracle_mod):
+    p = params_of(2, gravity=(0, 0, 0))
+    for vel, d in (([[-0.3, 0, 0], [0.3, 0, 0]], 0.4 * L0),  # separating
+                   ([[0.3, 0, 0], [-0.3, 0, 0]], 0.55 * L0)):  # beyond rlim = 0.5 l0
+        o = oracle_mod.Oracle(p, scene_of(2, [[0, 0, 0], [d, 0, 0]], vel=vel))
+        o.phase(ffi.PH_MKBKT)
+        o.phase(ffi.PH_COL)
+        assert np.array_equal(o.state()["vel"][:, :2], np.array(vel).T[:, :2])
+
+
+def test_wall_contact_radius(oracle_mod):
+    p = params_of(2, gravity=(0, 0, 0), restitution=0.0)
+    for d, hit in ((0.44 * L0, True), (0.46 * L0, False)):
+        o = oracle_mod.Oracle(p, scene_of(2, [[0, d, 0]], walls=[[0, 0, 0]], normals=[[0, 1, 0]],
+                                          vel=[[0, -0.2, 0]]))
+        o.phase(ffi.PH_MKBKT)
+        o.phase(ffi.PH_COL)
+        vy = o.state()["vel"][1, 0]
+        # wall mass = fluid mass: v' = v - (1+e)/2 * v
+        assert np.isclose(vy, -0.1 if hit else -0.2, rtol=1e-14)
+
+
+# --- MkPrs (kernel.cu:1306-1411) ----------------------------------------------------
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_lattice_interior_has_reference_density(oracle_mod, dim):
+    pts = lattice(dim, 4 if dim == 2 else 3)
+    o = oracle_mod.Oracle(params_of(dim), scene_of(dim, pts))
+    o.phase(ffi.PH_MKBKT)
+    o.phase(ffi.PH_PRS)
+    st = o.state()
+    i0 = idx_of(pts, [0, 0, 0])
+    n0g = o.constants()["n0_grad"]
+    assert abs(st["nd"][i0] - n0g) < 1e-12 * n0g
+    assert abs(st["prs"][i0]) < 1e-6
+    corner = idx_of(pts, [pts[:, 0].max(), pts[:, 1].max(), pts[:, 2].max()])
+    assert st["nd"][corner] < n0g and st["prs"][corner] == 0.0  # free surface: clipped at 0
+
+
+@pytest.mark.parametrize("s", [0.97, 0.93])
+def test_compressed_lattice_pressure(oracle_mod, s):
+    pts = lattice(2, 4, spacing=s * L0)
+    p = params_of(2, pmax_mode=ffi.PMAX_NONE)
+    o = oracle_mod.Oracle(p, scene_of(2, pts))
+    o.phase(ffi.PH_MKBKT)
+    o.phase(ffi.PH_PRS)
+    st = o.state()
+    c = o.constants()
+    i0 = idx_of(pts, [0, 0, 0])
+    rp = 2.1
+    n = sum(rp / (s * math.hypot(a, b)) - s * math.hypot(a, b) / rp
+            for a in range(-4, 5) for b in range(-4, 5)
+            if (a, b) != (0, 0) and s * math.hypot(a, b) < rp)
+    assert abs(st["nd"][i0] - n) < 1e-12 * n
+    P = 22.0**2 * 1000.0 * (n - c["n0_grad"]) / c["n0_grad"]
+    assert P > 0 and abs(st["prs"][i0] - P) < 1e-9 * P
+
+
+def test_pressure_clamp_modes(oracle_mod):
+    pts = lattice(2, 4, spacing=0.8 * L0) + np.array([0, 0.3, 0])
+    for mode, expect in ((ffi.PMAX_NONE, None), (ffi.PMAX_ABS_G, "cap"), (ffi.PMAX_REFERENCE, "neg")):
+        o = oracle_mod.Oracle(params_of(2, pmax_mode=mode), scene_of(2, pts))
+        o.phase(ffi.PH_MKBKT)
+        o.phase(ffi.PH_PRS)
+        c = o.constants()
+        prs = o.state()["prs"]
+        H = c["max_height"] - c["min_height"]
+        if expect is None:
+            assert math.isinf(c["pmax"]) and prs.max() > 1000.0 * 9.81 * H
+        elif expect == "cap":
+            assert c["pmax"] == pytest.approx(1000.0 * 9.81 * H) and prs.max() == c["pmax"]
+        else:  # kernel.cu:411 literally: Pmax < 0, every fluid pressure becomes Pmax
+            assert c["pmax"] < 0 and np.all(prs == c["pmax"])
+
+
+# --- PrsGrdTrm (kernel.cu:1651-1794) -----------------------------------------------
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_linear_pressure_gradient_is_exact_on_lattice(oracle_mod, dim):
+    pts = lattice(dim, 4 if dim == 2 else 3)
+    a = np.array([1200.0, -800.0, 500.0 if dim == 3 else 0.0])
+    o = oracle_mod.Oracle(params_of(dim), scene_of(dim, pts))
+    o.phase(ffi.PH_MKBKT)
+    o.set_state(prs=pts @ a + 5000.0)
+    o.phase(ffi.PH_GRAD)
+    acc = o.state()["acc"]
+    i0 = idx_of(pts, [0, 0, 0])
+    assert np.allclose(acc[:, i0], -a / 1000.0, rtol=1e-12, atol=1e-12)
+
+
+def test_gradient_uses_local_minimum_pressure(oracle_mod):
+    # a lone pair: only P_j - P_min enters; the lower-pressure particle feels nothing
+    p = params_of(2, gravity=(0, 0, 0))
+    o = oracle_mod.Oracle(p, scene_of(2, [[0, 0, 0], [L0, 0, 0]]))
+    o.phase(ffi.PH_MKBKT)
+    o.set_state(prs=np.array([100.0, 300.0]))
+    o.phase(ffi.PH_GRAD)
+    acc = o.state()["acc"]
+    n0g = o.constants()["n0_grad"]
+    w = (2.1 - 1 / 2.1) / L0**2
+    assert np.isclose(acc[0, 0], -(1 / 1000.0) * (2 / n0g) * 200.0 * w * L0, rtol=1e-13)
+    assert acc[0, 1] == 0.0  # P_j - P_min = 0 for the high-pressure particle's neighbour
+
+
+# --- mirrors (GenMRR_nonslip, MRRinout; kernel.cu:720-985) ----------------------------
+
+def test_plane_mirror_is_reflection_across_first_wall(oracle_mod):
+    x = np.array([0.003, 0.012, 0.0])
+    walls = [[w, -0.5 * L0, 0] for w in np.arange(-4, 5) * L0]
+    o = oracle_mod.Oracle(params_of(2), scene_of(2, [x], walls=walls, normals=[[0, 1, 0]] * len(walls)))
+    o.phase(ffi.PH_MKBKT)
+    o.phase(ffi.PH_MIRROR)
+    ids, mp, _ = o.mirrors()
+    assert list(ids) == [0]  # one distinct normal -> one mirror, slot 0 of particle 0
+    assert np.allclose(mp[:, 0], [x[0], 2 * (-0.5 * L0) - x[1], 0.0], rtol=0, atol=1e-17)
+
+
+def test_mirror_on_a_wall_particle_is_culled(oracle_mod):
+    # Both wall layers lie in one bucket row (y in [-0.042, -0.011)), the inner
+    # one listed first, so it is the first wall met.  The second layer sits
+    # exactly where the mirror lands: inout = 0 -> GST (kernel.cu:983).
+    x = [0.0, -0.5 * L0, 0.0]
+    walls = [[w, -1.5 * L0, 0] for w in np.arange(-4, 5) * L0] + [[w, -2.5 * L0, 0] for w in np.arange(-4, 5) * L0]
+    o = oracle_mod.Oracle(params_of(2), scene_of(2, [x], walls=walls, normals=[[0, 1, 0]] * len(walls)))
+    o.phase(ffi.PH_MKBKT)
+    o.phase(ffi.PH_MIRROR)
+    assert len(o.mirrors()[0]) == 0
+    x2 = [0.0, -0.38 * L0, 0.0]  # mirror at -2.62 l0: beyond the second layer -> kept
+    o = oracle_mod.Oracle(params_of(2), scene_of(2, [x2], walls=walls, normals=[[0, 1, 0]] * len(walls)))
+    o.phase(ffi.PH_MKBKT)
+    o.phase(ffi.PH_MIRROR)
+    ids, mp, _ = o.mirrors()
+    assert list(ids) == [0] and np.isclose(mp[1, 0], -2.62 * L0, rtol=1e-12)
+
+
+def test_corner_mirrors(oracle_mod):
+    sc = ffi.dambreak(dim=2, l0=L0, tank=(0.1, 0.1, 0), column=(0.01, 0.01, 0), wall_layers=1)
+    p = ffi.default_params(dim=2, precision="fp64", l0=L0, dt=1e-4, c=22.0, gravity=(0, -9.81, 0),
+                           range_min=sc.range_min, range_max=sc.range_max)
+    o = oracle_mod.Oracle(p, sc)
+    e = o.edge()
+    walls = sc.typ == ffi.WLL
+    corner = walls & (sc.pos[0] < 0) & (sc.pos[1] < 0)
+    assert e[corner].all()  # the corner site differs in normal from its neighbours
+    o.phase(ffi.PH_MIRROR)
+    ids, mp, _ = o.mirrors()
+    x = sc.pos[:, 0]
+    # three distinct normals (bottom, left, corner) -> slots 0..2, plus the edge mirror in slot 3
+    assert sorted(ids % 10) == [0, 1, 2, 3] and np.all(ids // 10 == 0)
+    got = {tuple(np.round(mp[:2, k] / L0, 9)) for k in range(len(ids))}
+    assert (round(x[0] / L0, 9), round(-2 * 0.5 - x[1] / L0, 9)) in got  # across y = -l0/2
+    assert (round(-2 * 0.5 - x[0] / L0, 9), round(x[1] / L0, 9)) in got  # across x = -l0/2
+
+
+# --- UpPcl1/2 and ChkPcl (kernel.cu:481-513, 1138-1149, 1815-1829) ---------------------
+
+def test_update_algebra_and_speed_clamp(oracle_mod):
+    p = params_of(2, dt=1e-3, gravity=(0, -9.81, 0))
+    # kernel.cu:401-406 scans heights with if / else-if: listed high first, the
+    # span is H = 0.4 and umax = sqrt(2 g H) ...
+    sc = scene_of(2, [[0.5, 0.4, 0], [0.0, 0.0, 0]], vel=[[0.0, 0.0, 0], [0.2, 0.1, 0]])
+    o = oracle_mod.Oracle(p, sc)
+    umax = o.constants()["umax"]
+    assert umax == pytest.approx(math.sqrt(2 * 9.81 * 0.4), rel=1e-15)
+    # ... listed low first, the minimum is never recorded and umax is NaN
+    assert math.isnan(oracle_mod.Oracle(p, scene_of(2, [[0.0, 0.0, 0], [0.5, 0.4, 0]])).constants()["umax"])
+    o.phase(ffi.PH_MKBKT)
+    o.phase(ffi.PH_VSC)
+    o.phase(ffi.PH_UP1)
+    st = o.state()
+    v = np.array([0.2, 0.1 - 9.81e-3])
+    assert np.array_equal(st["vel"][:2, 1], v)
+    assert np.array_equal(st["pos"][:2, 1], np.array([0.0, 0.0]) + v * 1e-3)
+    assert np.all(st["acc"] == 0)
+    o.set_state(vel=np.array([[0.0, 10.0], [0.0, 0.0], [0.0, 0.0]]))
+    o.phase(ffi.PH_VSC)
+    o.phase(ffi.PH_UP1)
+    vel = o.state()["vel"][:, 1]
+    assert np.isclose(np.hypot(vel[0], vel[1]), umax, rtol=1e-14)
+
+
+def test_particle_leaving_the_domain_is_deleted(oracle_mod):
+    p = params_of(2, dt=1e-3, gravity=(0, 0, 0))
+    o = oracle_mod.Oracle(p, scene_of(2, [[0.999, 0.0, 0], [0.0, 0.0, 0]], vel=[[2.0, 0, 0], [0, 0, 0]]))
+    o.set_state(prs=np.array([123.0, 0.0]))
+    o.phase(ffi.PH_MKBKT)
+    o.phase(ffi.PH_VSC)
+    o.phase(ffi.PH_UP1)
+    st = o.state()
+    assert st["typ"][0] == ffi.GST and np.all(st["pos"][:, 0] == 0) and np.all(st["vel"][:, 0] == 0)
+    assert st["prs"][0] == 0 and st["typ"][1] == ffi.FLD
+
+
+# --- whole-step behaviour -----------------------------------------------------------
+
+def test_step_is_the_reference_phase_sequence(oracle_mod):
+    sc, p = configs.small2d("fp64")
+    a, b = oracle_mod.Oracle(p, sc), oracle_mod.Oracle(p, sc)
+    a.step(3)
+    for _ in range(3):
+        for ph in ffi.STEP_PHASES:
+            b.phase(ph)
+    assert_states_equal(a.state(), b.state(), "step vs phases")
+
+
+def test_oracle_is_deterministic_across_thread_counts(oracle_mod):
+    sc, p = configs.small3d("fp64")
+    old = oracle_mod.set_threads(0)
+    try:
+        oracle_mod.set_threads(1)
+        a = oracle_mod.Oracle(p, sc)
+        a.step(20)
+        oracle_mod.set_threads(4)
+        b = oracle_mod.Oracle(p, sc)
+        b.step(20)
+    finally:
+        oracle_mod.set_threads(old)
+    assert_states_equal(a.state(), b.state(), "1 vs 4 threads")
+
+
+def test_fp32_instantiation_tracks_fp64(oracle_mod):
+    sc, p32 = configs.small3d("fp32")
+    a = oracle_mod.Oracle(p32, sc)
+    sc.pos = a.state()["pos"]  # identical FP32-representable start
+    _, p64 = configs.small3d("fp64")
+    b = oracle_mod.Oracle(p64, sc)
+    a.step(1)
+    b.step(1)
+    sa, sb = a.state(), b.state()
+    live = sb["typ"] == ffi.FLD
+    assert np.array_equal(sa["typ"], sb["typ"])
+    assert np.array_equal(a.mirrors()[0], b.mirrors()[0])  # no discrete decision differs
+    # Positions agree to FP32 rounding.  Pressure (and through it velocity)
+    # amplifies the FP32 position quantum by c^2 rho / n0 ~ 2e4: 1e-4 level.
+    assert rel_inf(sa["pos"], sb["pos"], live) < 1e-6
+    assert rel_inf(sa["vel"], sb["vel"], live) < 2e-4
+    assert rel_inf(sa["prs"], sb["prs"], live) < 2e-4
+
+
+def test_dam_break_stays_in_the_tank(oracle_mod):
+    sc, p = configs.cfg0("fp64")
+    o = oracle_mod.Oracle(p, sc)
+    o.step(400)  # 0.2 s
+    st = o.state()
+    live = st["typ"] == ffi.FLD
+    assert live.sum() >= 1245
+    front = st["pos"][0][live].max()
+    assert 0.4 < front < 0.8  # the surge travels right, about 2 m/s
+    assert st["pos"][1][live].min() > -0.01
