@@ -1,0 +1,160 @@
+#!/usr/bin/env python3
+"""Throughput of the explicit MPS step on one MI355X.
+
+Workload: the 3-D dam break with 300,000 fluid particles in FP32 (tank
+1.0 x 0.6 x 0.3 m, l0 = 5 mm, 3 wall layers, dt = 2.5e-4 s).  A step is one
+pass of the ClcMPS loop body over the resident state (inputs are in HBM
+before the timed region).  Prints one JSON line:
+
+  value         live fluid particles x steps / second (particle-steps/s)
+  roofline      the dominant kernel's algorithmic HBM bytes per launch over
+                its average launch time (HIP events on the handle's stream),
+                against 8 TB/s; "valu" gives the same kernel's candidate-test
+                flops against the FP32 VALU peak
+  cpu_baseline  the FP32 CPU oracle on this host's cores, on a bounded
+                sample of the same workload (first steps of the same scene)
+
+Usage: python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+FLOP_PER_CANDIDATE = 9  # 3 sub, 3 mul, 2 add, 1 compare per candidate pair
+
+METRIC = "particle-steps/s, 3D dam break 300k FP32"
+
+
+def algorithmic_bytes(kernel: str, nf: int, nw: int, m: int) -> float:
+    """Compulsory HBM bytes of one launch: every array the phase reads or
+    writes, counted once, at 12 B per FP32 3-vector, 4 B per scalar."""
+    table = {
+        "vsc_up1": 49 * nf + 12 * m,       # pos, vel in; pos, vel out; typ
+        "col": 37 * nf + 13 * nw + 12 * m,
+        "prs": 17 * nf + 12 * m,
+        "grad_up2": 61 * nf + 16 * m,
+        "mirror_gen": 13 * nf + 26 * nw + 29 * m,
+    }
+    return float(table.get(kernel, 0))
+
+
+def candidates_per_pass(h) -> int:
+    """Fluid candidates the 27-cell stencil visits, summed over live fluid
+    particles (one pass of a fluid neighbour loop)."""
+    c = h.constants()
+    cs, ids = h.buckets(0)
+    nb = c["nb"]
+    occ = np.diff(cs).reshape(nb[2], nb[1], nb[0]).astype(np.int64)
+    p = np.pad(occ, 1)
+    box = sum(p[1 + dz:1 + dz + nb[2], 1 + dy:1 + dy + nb[1], 1 + dx:1 + dx + nb[0]]
+              for dz in (-1, 0, 1) for dy in (-1, 0, 1) for dx in (-1, 0, 1))
+    return int((occ * box).sum())
+
+
+def cpu_baseline(seconds: float) -> dict:
+    from mps_cuda_amd import configs
+    from oracle import pyoracle
+    threads = pyoracle.set_threads(0)
+    scene, params = configs.cfg1("fp32")
+    o = pyoracle.Oracle(params, scene)
+    nf = int((scene.typ == 0).sum())
+    t0 = time.perf_counter()
+    o.step(1)
+    t1 = time.perf_counter() - t0
+    n = max(1, int(seconds / max(t1, 1e-6)))
+    t0 = time.perf_counter()
+    o.step(n)
+    dt = time.perf_counter() - t0
+    live = int((o.state()["typ"] == 0).sum())
+    return {"value": live * n / dt, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": f"FP32 CPU oracle (OpenMP), same 300k scene, steps 2-{n + 1} ({n} steps, "
+                      f"{dt:.1f} s, {nf} fluid at start)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only: this build has no multi-GPU path (asked for {args.gpus})",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from mps_cuda_amd import configs, ffi
+
+    scene, params = configs.cfg1("fp32")
+    nf0 = int((scene.typ == ffi.FLD).sum())
+    nw = int((scene.typ == ffi.WLL).sum())
+    h = ffi.Handle(params, scene)
+    print(f"[bench] cfg1: {nf0} fluid, {nw} walls; warmup {args.warmup}", file=sys.stderr, flush=True)
+    h.step(args.warmup)
+    h.sync()
+    live0 = int((h.state()["typ"] == ffi.FLD).sum())
+    t0 = time.perf_counter()
+    h.step(args.steps)
+    h.sync()
+    elapsed = time.perf_counter() - t0
+    live1 = int((h.state()["typ"] == ffi.FLD).sum())
+    live = 0.5 * (live0 + live1)
+    value = live * args.steps / elapsed
+    print(f"[bench] {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
+
+    # per-kernel times over a separate timed window of the same length
+    nt = max(10, min(args.steps, 100))
+    h.timing(True)
+    h.step(nt)
+    kt = h.timing_read()
+    h.timing(False)
+    m = len(h.mirrors()[0])
+    dom = max((k for k in kt if algorithmic_bytes(k, 1, 1, 1) > 0), key=lambda k: kt[k][0])
+    avg_ms = kt[dom][0] / kt[dom][1]
+    nlive = int((h.state()["typ"] == ffi.FLD).sum())
+    bytes_ = algorithmic_bytes(dom, nlive, nw, m)
+    achieved = bytes_ / (avg_ms * 1e-3) / 1e9
+    passes = {"vsc_up1": 1, "prs": 1, "grad_up2": 2, "col": 1}.get(dom, 1)
+    flops = FLOP_PER_CANDIDATE * candidates_per_pass(h) * passes
+    tflops = flops / (avg_ms * 1e-3) / 1e12
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "bytes_per_launch": bytes_, "avg_ms": round(avg_ms, 5),
+                "valu": {"achieved": round(tflops, 2), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": round(tflops / FP32_PEAK_TFLOPS, 4), "flops_per_launch": flops}}
+
+    cpu = None
+    if not args.no_cpu:
+        print("[bench] CPU baseline ...", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(args.cpu_seconds)
+
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "particle-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * elapsed / args.steps, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "cfg1: 3-D dam break, 300k fluid, FP32", "fluid": nf0, "walls": nw,
+                   "live_fluid": live, "mirrors": m, "l0": 0.005, "dt": 2.5e-4,
+                   "tank": [1.0, 0.6, 0.3], "column": [0.25, 0.5, 0.3]},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "kernels_ms": {k: round(v[0] / v[1], 5) for k, v in sorted(kt.items(), key=lambda kv: -kv[1][0])},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

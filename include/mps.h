@@ -213,6 +213,19 @@ int mps_get_mirrors(mps_handle* h, int* ids, double* pos, double* prs, int* coun
 
 const char* mps_last_error(void);
 
+/* ---- measurement (no reference counterpart; the reference only times the
+ * whole run, main.cu:4-5, 28-36) --------------------------------------- */
+
+/* on != 0: bracket every hot kernel launch with HIP events on the handle's
+ * stream and reset the accumulators; on == 0: stop. */
+int mps_timing_enable(mps_handle* h, int on);
+
+/* Accumulated kernel times since mps_timing_enable: up to `max` entries of
+ * names (48 chars each), total milliseconds and launch counts; *n_out
+ * receives the number of kernel names. */
+int mps_timing_read(mps_handle* h, int max, char* names, double* total_ms, long long* counts,
+                    int* n_out);
+
 /* ---- host-only helpers (no device needed) ----------------------------- */
 
 /* SetPara + AlcBkt constants for a scene without creating a handle

@@ -70,8 +70,75 @@ struct ImplBase {
   void free_all() {
     for (void* p : allocs) (void)hipFree(p);
     allocs.clear();
+    for (auto& r : recs) pool.push_back(r.a), pool.push_back(r.b);
+    recs.clear();
+    for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+    pool.clear();
+  }
+
+  // ---- optional per-kernel timing with HIP events on the handle's stream ----
+  struct Rec {
+    int name;
+    hipEvent_t a, b;
+  };
+  bool timing = false;
+  std::vector<Rec> recs;
+  std::vector<hipEvent_t> pool;
+  std::vector<std::string> tnames;
+  std::vector<double> tms;
+  std::vector<long long> tcount;
+  hipEvent_t event() {
+    if (pool.empty()) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return nullptr;
+      return e;
+    }
+    hipEvent_t e = pool.back();
+    pool.pop_back();
+    return e;
+  }
+  void tick(const char* nm) {
+    if (!timing) return;
+    int id = -1;
+    for (size_t k = 0; k < tnames.size(); k++)
+      if (tnames[k] == nm) id = (int)k;
+    if (id < 0) {
+      id = (int)tnames.size();
+      tnames.push_back(nm);
+      tms.push_back(0.0);
+      tcount.push_back(0);
+    }
+    Rec r{id, event(), event()};
+    (void)hipEventRecord(r.a, stream);
+    recs.push_back(r);
+  }
+  void tock() {
+    if (!timing || recs.empty()) return;
+    (void)hipEventRecord(recs.back().b, stream);
+    if (recs.size() >= 8192) (void)flush_timing();
+  }
+  int flush_timing() {
+    if (recs.empty()) return MPS_OK;
+    HIP_TRY(hipStreamSynchronize(stream));
+    for (auto& r : recs) {
+      float ms = 0.f;
+      HIP_TRY(hipEventElapsedTime(&ms, r.a, r.b));
+      tms[r.name] += ms;
+      tcount[r.name] += 1;
+      pool.push_back(r.a);
+      pool.push_back(r.b);
+    }
+    recs.clear();
+    return MPS_OK;
   }
 };
+
+#define TLAUNCH(name, ...)          \
+  do {                              \
+    tick(name);                     \
+    hipLaunchKernelGGL(__VA_ARGS__); \
+    tock();                         \
+  } while (0)
 
 template <class real, int DIM>
 struct Impl : ImplBase {
@@ -255,6 +322,7 @@ struct Impl : ImplBase {
       HIP_TRY(hipMemcpy(fpos[0], hp.data(), nf * sizeof(V), hipMemcpyHostToDevice));
       HIP_TRY(hipMemcpy(fvel[0], hv.data(), nf * sizeof(V), hipMemcpyHostToDevice));
       HIP_TRY(hipMemcpy(facc[0], ha.data(), nf * sizeof(V), hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(facc[1], ha.data(), nf * sizeof(V), hipMemcpyHostToDevice));
       HIP_TRY(hipMemcpy(fprs[0], hprs.data(), nf * sizeof(real), hipMemcpyHostToDevice));
       HIP_TRY(hipMemcpy(fpav[0], hpav.data(), nf * sizeof(real), hipMemcpyHostToDevice));
       HIP_TRY(hipMemcpy(fnd[0], hnd.data(), nf * sizeof(real), hipMemcpyHostToDevice));
@@ -289,7 +357,7 @@ struct Impl : ImplBase {
   void swap_all() {
     std::swap(fpos[0], fpos[1]);
     std::swap(fvel[0], fvel[1]);
-    std::swap(facc[0], facc[1]);
+    if (acc_live) std::swap(facc[0], facc[1]);  // gathered only while live
     std::swap(fprs[0], fprs[1]);
     std::swap(fpav[0], fpav[1]);
     std::swap(fnd[0], fnd[1]);
@@ -306,14 +374,16 @@ struct Impl : ImplBase {
     }
     const dim3 B(mps::kBlock);
     hipLaunchKernelGGL(mps::k_fill_int, dim3(nblk(ncell + 1, 4096)), B, 0, stream, cnt, ncell + 1, 0);
-    hipLaunchKernelGGL((mps::k_bkt_key<real, DIM>), dim3(nblk(nf)), B, 0, stream, fpos[0],
+    TLAUNCH("bucket_key", (mps::k_bkt_key<real, DIM>), dim3(nblk(nf)), B, 0, stream, fpos[0],
                        (const int8_t*)ftyp[0], nf, (const int*)nullptr, c, key, slot, cnt);
+    tick("bucket_scan");
     scan(cnt, ncell + 1, fcs, 0);
-    hipLaunchKernelGGL(mps::k_bkt_scatter, dim3(nblk(nf)), B, 0, stream, key, slot, fcs, nf,
+    tock();
+    TLAUNCH("bucket_scatter", mps::k_bkt_scatter, dim3(nblk(nf)), B, 0, stream, key, slot, fcs, nf,
                        (const int*)nullptr, tmp);
-    hipLaunchKernelGGL(mps::k_bkt_rank, dim3(nblk(nf)), B, 0, stream, tmp, key, fid[0], fcs, nf,
+    TLAUNCH("bucket_rank", mps::k_bkt_rank, dim3(nblk(nf)), B, 0, stream, tmp, key, fid[0], fcs, nf,
                        (const int*)nullptr, ncell, perm);
-    hipLaunchKernelGGL((mps::k_gather_fluid<real, DIM>), dim3(nblk(nf)), B, 0, stream, perm, nf,
+    TLAUNCH("bucket_gather", (mps::k_gather_fluid<real, DIM>), dim3(nblk(nf)), B, 0, stream, perm, nf,
                        fa(0), fa(1));
     swap_all();
     return MPS_OK;
@@ -324,19 +394,21 @@ struct Impl : ImplBase {
     const dim3 B(mps::kBlock);
     hipLaunchKernelGGL(mps::k_fill_int, dim3(1), B, 0, stream, mcount, 1, 0);
     if (nf > 0)
-      hipLaunchKernelGGL((mps::k_mirror_gen<real, DIM>), dim3(nblk(nf)), B, 0, stream, fpos[0],
+      TLAUNCH("mirror_gen", (mps::k_mirror_gen<real, DIM>), dim3(nblk(nf)), B, 0, stream, fpos[0],
                          fprs[0], fid[0], fcs, nf, wpos, wnrm, wedge, wcs, c, mraw_pos, mraw_prs,
                          mraw_id, mcount, mcap);
     const int g = nblk(mcap, 2048);
     hipLaunchKernelGGL(mps::k_fill_int, dim3(nblk(ncell + 1, 4096)), B, 0, stream, cnt, ncell + 1, 0);
-    hipLaunchKernelGGL((mps::k_bkt_key<real, DIM>), dim3(g), B, 0, stream, mraw_pos,
+    TLAUNCH("mirror_key", (mps::k_bkt_key<real, DIM>), dim3(g), B, 0, stream, mraw_pos,
                        (const int8_t*)nullptr, 0, (const int*)mcount, c, key, slot, cnt);
+    tick("mirror_scan");
     scan(cnt, ncell + 1, mcs, 0);
-    hipLaunchKernelGGL(mps::k_bkt_scatter, dim3(g), B, 0, stream, key, slot, mcs, 0,
+    tock();
+    TLAUNCH("mirror_scatter", mps::k_bkt_scatter, dim3(g), B, 0, stream, key, slot, mcs, 0,
                        (const int*)mcount, tmp);
-    hipLaunchKernelGGL(mps::k_bkt_rank, dim3(g), B, 0, stream, tmp, key, mraw_id, mcs, 0,
+    TLAUNCH("mirror_rank", mps::k_bkt_rank, dim3(g), B, 0, stream, tmp, key, mraw_id, mcs, 0,
                        (const int*)mcount, ncell, perm);
-    hipLaunchKernelGGL((mps::k_gather_mirror<real, DIM>), dim3(g), B, 0, stream, perm,
+    TLAUNCH("mirror_gather", (mps::k_gather_mirror<real, DIM>), dim3(g), B, 0, stream, perm,
                        (const int*)mcount, mraw_pos, mraw_prs, mraw_id, mpos, mprs, mid);
     return MPS_OK;
   }
@@ -361,22 +433,22 @@ struct Impl : ImplBase {
       bucket();
       mirror();
       if (nf > 0) {
-        hipLaunchKernelGGL((mps::k_vsc<real, DIM, true>), G, B, 0, stream, fpos[0], fvel[0], fcs, nf,
+        TLAUNCH("vsc_up1", (mps::k_vsc<real, DIM, true>), G, B, 0, stream, fpos[0], fvel[0], fcs, nf,
                            mpos, mcs, c, (V*)nullptr, fpos[1], fvel[1], ftyp[0], fprs[0]);
         std::swap(fpos[0], fpos[1]);
         std::swap(fvel[0], fvel[1]);
       }
       bucket();
       if (nf > 0) {
-        hipLaunchKernelGGL((mps::k_col<real, DIM, false>), G, B, 0, stream, fpos[0], fvel[0], fcs, nf,
+        TLAUNCH("col", (mps::k_col<real, DIM, false>), G, B, 0, stream, fpos[0], fvel[0], fcs, nf,
                            wpos, wvel, wcs, mpos, mcs, c, fvel[1], (V*)nullptr);
         std::swap(fvel[0], fvel[1]);
-        hipLaunchKernelGGL((mps::k_prs<real, DIM>), G, B, 0, stream, fpos[0], fcs, nf, mpos, mcs, c,
+        TLAUNCH("prs", (mps::k_prs<real, DIM>), G, B, 0, stream, fpos[0], fcs, nf, mpos, mcs, c,
                            fprs[0], fnd[0]);
       }
       mirror();
       if (nf > 0) {
-        hipLaunchKernelGGL((mps::k_grad<real, DIM, true>), G, B, 0, stream, fpos[0], fvel[0], fprs[0],
+        TLAUNCH("grad_up2", (mps::k_grad<real, DIM, true>), G, B, 0, stream, fpos[0], fvel[0], fprs[0],
                            fcs, nf, mpos, mprs, mcs, c, (V*)nullptr, fpos[1], fvel[1], fprs[1],
                            fpav[0], ftyp[0]);
         std::swap(fpos[0], fpos[1]);
@@ -682,6 +754,36 @@ int mps_get_buckets(mps_handle* h, int which, int* cell_start, int* ids, int* co
 int mps_get_mirrors(mps_handle* h, int* ids, double* pos, double* prs, int* count) {
   if (!h || !count) return set_err(MPS_E_ARG, "bad argument");
   return h->impl->mirrors(ids, pos, prs, count);
+}
+
+int mps_timing_enable(mps_handle* h, int on) {
+  if (!h) return set_err(MPS_E_ARG, "null handle");
+  ImplBase& b = *h->impl;
+  int rc = b.flush_timing();
+  if (rc) return rc;
+  b.timing = on != 0;
+  std::fill(b.tms.begin(), b.tms.end(), 0.0);
+  std::fill(b.tcount.begin(), b.tcount.end(), 0);
+  return MPS_OK;
+}
+
+int mps_timing_read(mps_handle* h, int max, char* names, double* total_ms, long long* counts,
+                    int* n_out) {
+  if (!h || !n_out || max < 0) return set_err(MPS_E_ARG, "bad argument");
+  ImplBase& b = *h->impl;
+  int rc = b.flush_timing();
+  if (rc) return rc;
+  const int n = (int)b.tnames.size();
+  *n_out = n;
+  for (int k = 0; k < n && k < max; k++) {
+    if (names) {
+      std::memset(names + 48 * k, 0, 48);
+      std::strncpy(names + 48 * k, b.tnames[k].c_str(), 47);
+    }
+    if (total_ms) total_ms[k] = b.tms[k];
+    if (counts) counts[k] = b.tcount[k];
+  }
+  return MPS_OK;
 }
 
 int mps_compute_constants(const mps_params* p, const mps_scene* s, mps_constants* out) {

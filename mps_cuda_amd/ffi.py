@@ -92,8 +92,13 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.mps_last_error.restype = C.c_char_p
     for fn in ("mps_scene_dambreak_count", "mps_scene_dambreak_fill", "mps_params_default",
                "mps_compute_constants", "mps_compute_edge", "mps_probe_arith", "mps_sync",
-               "mps_set_stream"):
+               "mps_set_stream", "mps_timing_enable", "mps_timing_read"):
         getattr(lib, fn).restype = C.c_int
+    lib.mps_sync.argtypes = [C.c_void_p]
+    lib.mps_set_stream.argtypes = [C.c_void_p, C.c_void_p]
+    lib.mps_timing_enable.argtypes = [C.c_void_p, C.c_int]
+    lib.mps_timing_read.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                    C.c_void_p]
     if path == LIB_PATH:
         _LIB = lib
     return lib
@@ -259,6 +264,27 @@ class Handle:
         ids = np.zeros(max(cnt.value, 1), np.int32)
         self._check(self._f("get_buckets")(self._h, which, cs.ctypes.data, ids.ctypes.data, C.byref(cnt)))
         return cs, ids[: cnt.value]
+
+    def sync(self):
+        self._check(self.lib.mps_sync(self._h))
+
+    def timing(self, on: bool):
+        """Per-kernel HIP-event timing on the handle's stream (HIP library only)."""
+        self._check(self.lib.mps_timing_enable(self._h, 1 if on else 0))
+
+    def timing_read(self) -> dict:
+        """{kernel: (total_ms, launches)} since timing(True)."""
+        n = C.c_int()
+        self._check(self.lib.mps_timing_read(self._h, 0, None, None, None, C.byref(n)))
+        k = n.value
+        names = C.create_string_buffer(48 * max(k, 1))
+        ms = np.zeros(max(k, 1))
+        cnt = np.zeros(max(k, 1), np.int64)
+        self._check(self.lib.mps_timing_read(self._h, k, names, C.c_void_p(ms.ctypes.data),
+                                             C.c_void_p(cnt.ctypes.data), C.byref(n)))
+        raw = names.raw
+        return {raw[48 * i:48 * i + 48].split(b"\0")[0].decode(): (float(ms[i]), int(cnt[i]))
+                for i in range(k)}
 
     def mirrors(self):
         cnt = C.c_int()
