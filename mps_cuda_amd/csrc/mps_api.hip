@@ -160,12 +160,15 @@ struct Impl : ImplBase {
   // walls (static)
   V *wpos = nullptr, *wvel = nullptr, *wnrm = nullptr;
   int8_t* wedge = nullptr;
+  uint8_t* wcls = nullptr;  // normal class (bits 0-5) | edge flag (bit 7)
+  bool use_cls = false;     // at most 64 distinct wall normals
   int* wcs = nullptr;
   real *wprs = nullptr, *wpav = nullptr;
   // mirrors
   V *mraw_pos = nullptr, *mpos = nullptr;
   real *mraw_prs = nullptr, *mprs = nullptr;
   int *mraw_id = nullptr, *mid = nullptr, *mcs = nullptr, *mcount = nullptr;
+  int* near_list = nullptr;  // fluid particles with walls in their stencil
   // bucket scratch (shared by the fluid and mirror builds)
   int *key = nullptr, *slot = nullptr, *tmp = nullptr, *perm = nullptr, *cnt = nullptr;
   // scan levels: parts[l] holds block totals, parts_ex[l] their exclusive scan
@@ -191,6 +194,8 @@ struct Impl : ImplBase {
     c.rp2 = hk.rp2;
     c.rlim2 = hk.rlim2;
     c.rlim_wall2 = hk.rlim_wall2;
+    c.r2_hi = hk.r2 * (real(1) + real(1.0 / 1024));
+    c.rp2_hi = hk.rp2 * (real(1) + real(1.0 / 1024));
     c.col = hk.col;
     c.mass = hk.mass;
     c.n0knm = (real(1) / hk.n0) * hk.nu;   // kernel.cu:1029, 1113
@@ -249,12 +254,14 @@ struct Impl : ImplBase {
     }
     if ((rc = alloc(&fcs, (size_t)ncell + 2))) return rc;
     if ((rc = alloc(&wpos, nw)) || (rc = alloc(&wvel, nw)) || (rc = alloc(&wnrm, nw)) ||
-        (rc = alloc(&wedge, nw)) || (rc = alloc(&wprs, nw)) || (rc = alloc(&wpav, nw)) ||
+        (rc = alloc(&wedge, nw)) || (rc = alloc(&wcls, nw)) || (rc = alloc(&wprs, nw)) ||
+        (rc = alloc(&wpav, nw)) ||
         (rc = alloc(&wcs, (size_t)ncell + 1)))
       return rc;
     if ((rc = alloc(&mraw_pos, mcap)) || (rc = alloc(&mpos, mcap)) || (rc = alloc(&mraw_prs, mcap)) ||
         (rc = alloc(&mprs, mcap)) || (rc = alloc(&mraw_id, mcap)) || (rc = alloc(&mid, mcap)) ||
-        (rc = alloc(&mcs, (size_t)ncell + 2)) || (rc = alloc(&mcount, 1)))
+        (rc = alloc(&mcs, (size_t)ncell + 2)) || (rc = alloc(&mcount, 1)) ||
+        (rc = alloc(&near_list, std::max(nf, 1))))
       return rc;
     const size_t scr = std::max(mcap, nf);
     if ((rc = alloc(&key, scr)) || (rc = alloc(&slot, scr)) || (rc = alloc(&tmp, scr)) ||
@@ -300,6 +307,21 @@ struct Impl : ImplBase {
       HIP_TRY(hipMemcpy(wprs, hprs.data(), nw * sizeof(real), hipMemcpyHostToDevice));
       HIP_TRY(hipMemcpy(wpav, hpav.data(), nw * sizeof(real), hipMemcpyHostToDevice));
       HIP_TRY(hipMemcpy(wcs, hcs.data(), (ncell + 1) * sizeof(int), hipMemcpyHostToDevice));
+      // normal classes: walls whose normals compare equal (as the reference's
+      // == on each component, kernel.cu:762) share a class
+      std::vector<V> cls_n;
+      std::vector<uint8_t> hcls(nw);
+      use_cls = true;
+      for (int w = 0; w < nw && use_cls; w++) {
+        int k = 0;
+        while (k < (int)cls_n.size() && !(cls_n[k].x == hn[w].x && cls_n[k].y == hn[w].y &&
+                                          mps::vz(cls_n[k]) == mps::vz(hn[w])))
+          k++;
+        if (k == (int)cls_n.size()) cls_n.push_back(hn[w]);
+        if (k >= 64) use_cls = false;
+        hcls[w] = uint8_t(k & 63) | (he[w] == MPS_EDGE ? 0x80 : 0);
+      }
+      HIP_TRY(hipMemcpy(wcls, hcls.data(), nw, hipMemcpyHostToDevice));
     }
     // upload the fluid set in creation order, then bucket it (ClcMPS starts
     // with MkBkt, kernel.cu:1878)
@@ -380,9 +402,9 @@ struct Impl : ImplBase {
     scan(cnt, ncell + 1, fcs, 0);
     tock();
     TLAUNCH("bucket_scatter", mps::k_bkt_scatter, dim3(nblk(nf)), B, 0, stream, key, slot, fcs, nf,
-                       (const int*)nullptr, tmp);
+                       (const int*)nullptr, 1, tmp);
     TLAUNCH("bucket_rank", mps::k_bkt_rank, dim3(nblk(nf)), B, 0, stream, tmp, key, fid[0], fcs, nf,
-                       (const int*)nullptr, ncell, perm);
+                       (const int*)nullptr, 1, ncell, perm);
     TLAUNCH("bucket_gather", (mps::k_gather_fluid<real, DIM>), dim3(nblk(nf)), B, 0, stream, perm, nf,
                        fa(0), fa(1));
     swap_all();
@@ -393,23 +415,34 @@ struct Impl : ImplBase {
   int mirror() {
     const dim3 B(mps::kBlock);
     hipLaunchKernelGGL(mps::k_fill_int, dim3(1), B, 0, stream, mcount, 1, 0);
-    if (nf > 0)
+    // raw slots: 10 per listed particle (wave path) or compacted (general path)
+    int mult = 1;
+    if (nf > 0 && use_cls) {
+      TLAUNCH("mirror_near", (mps::k_mirror_near<real, DIM>), dim3(nblk(nf)), B, 0, stream, fpos[0],
+              fcs, nf, wcs, c, near_list, mcount);
+      TLAUNCH("mirror_gen", (mps::k_mirror_wave<real, DIM>), dim3(std::min(nf, 16384)), dim3(64), 0,
+              stream, fpos[0], fprs[0], fid[0], near_list, mcount, wpos, wnrm, wcls, wcs, c,
+              mraw_pos, mraw_prs, mraw_id);
+      mult = mps::kNumMRR;
+    } else if (nf > 0) {
       TLAUNCH("mirror_gen", (mps::k_mirror_gen<real, DIM>), dim3(nblk(nf)), B, 0, stream, fpos[0],
-                         fprs[0], fid[0], fcs, nf, wpos, wnrm, wedge, wcs, c, mraw_pos, mraw_prs,
-                         mraw_id, mcount, mcap);
+              fprs[0], fid[0], fcs, nf, wpos, wnrm, wedge, wcs, c, mraw_pos, mraw_prs, mraw_id,
+              mcount, mcap);
+    }
     const int g = nblk(mcap, 2048);
     hipLaunchKernelGGL(mps::k_fill_int, dim3(nblk(ncell + 1, 4096)), B, 0, stream, cnt, ncell + 1, 0);
-    TLAUNCH("mirror_key", (mps::k_bkt_key<real, DIM>), dim3(g), B, 0, stream, mraw_pos,
-                       (const int8_t*)nullptr, 0, (const int*)mcount, c, key, slot, cnt);
+    TLAUNCH("mirror_key", (mps::k_mirror_key<real, DIM>), dim3(g), B, 0, stream, mraw_pos,
+            (const int*)mraw_id, (const int*)mcount, mult, c, key, slot, cnt);
     tick("mirror_scan");
     scan(cnt, ncell + 1, mcs, 0);
     tock();
     TLAUNCH("mirror_scatter", mps::k_bkt_scatter, dim3(g), B, 0, stream, key, slot, mcs, 0,
-                       (const int*)mcount, tmp);
+            (const int*)mcount, mult, tmp);
     TLAUNCH("mirror_rank", mps::k_bkt_rank, dim3(g), B, 0, stream, tmp, key, mraw_id, mcs, 0,
-                       (const int*)mcount, ncell, perm);
+            (const int*)(mcs + ncell), 1, ncell, perm);
+    // live mirrors: mcs[ncell] (empty slots sit in the tail cell)
     TLAUNCH("mirror_gather", (mps::k_gather_mirror<real, DIM>), dim3(g), B, 0, stream, perm,
-                       (const int*)mcount, mraw_pos, mraw_prs, mraw_id, mpos, mprs, mid);
+            (const int*)(mcs + ncell), mraw_pos, mraw_prs, mraw_id, mpos, mprs, mid);
     return MPS_OK;
   }
 

@@ -54,6 +54,10 @@ template <class real>
 struct Cst {
   real minc[3], maxc[3], lo[3], hi[3], g[3];
   real dbinv, l0, r, r2, rp, rp2, rlim2, rlim_wall2;
+  // r2_hi, rp2_hi: squared radii enlarged by 2^-10 relative.  sqrt is
+  // correctly rounded, so sqrt(d2) <= r implies d2 <= r2_hi: candidates
+  // beyond them skip the sqrt and the exact test without changing a result.
+  real r2_hi, rp2_hi;
   real col, mass, n0knm, grad_a, invro, dt, umax, prs_coef, n0_grad, pmax;
   int nb[3], nbxy, ncell;
 };

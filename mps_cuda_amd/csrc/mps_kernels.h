@@ -106,12 +106,15 @@ __global__ void k_bkt_key(const Vec<real, DIM>* __restrict__ pos, const int8_t* 
   }
 }
 
+// Element count: n_static, or mult * (*n_dyn) when n_dyn is given.
 __global__ void k_bkt_scatter(const int* __restrict__ key, const int* __restrict__ slot,
                               const int* __restrict__ cs, int n_static, const int* __restrict__ n_dyn,
-                              int* __restrict__ tmp) {
-  const int n = n_dyn ? *n_dyn : n_static;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    tmp[cs[key[i]] + slot[i]] = i;
+                              int mult, int* __restrict__ tmp) {
+  const int n = n_dyn ? *n_dyn * mult : n_static;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int k = key[i];
+    if (k >= 0) tmp[cs[k] + slot[i]] = i;
+  }
 }
 
 // Final position of an element = start of its cell + number of elements of
@@ -119,8 +122,9 @@ __global__ void k_bkt_scatter(const int* __restrict__ key, const int* __restrict
 // scatter order: its order is never observed.
 __global__ void k_bkt_rank(const int* __restrict__ tmp, const int* __restrict__ key,
                            const int* __restrict__ id, const int* __restrict__ cs, int n_static,
-                           const int* __restrict__ n_dyn, int ncell, int* __restrict__ perm) {
-  const int n = n_dyn ? *n_dyn : n_static;
+                           const int* __restrict__ n_dyn, int mult, int ncell,
+                           int* __restrict__ perm) {
+  const int n = n_dyn ? *n_dyn * mult : n_static;
   for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
     const int i = tmp[s];
     const int k = key[i];
@@ -184,13 +188,10 @@ __global__ void k_gather_mirror(const int* __restrict__ perm, const int* __restr
 // d_MRRinout (kernel.cu:932-985) for each generated mirror; survivors are
 // appended to the raw mirror set.
 
-struct WallArraysC {
-  const void* pos;
-  const void* nrm;
-  const int8_t* edge;
-  const int* cs;
-};
-
+// Nearest wall particle within r of a mirror; the first met wins a tie
+// (kernel.cu:965-971).  A candidate can only pass `dist <= r && dist <
+// walldist` when d2 <= r2_hi and d2 <= d2 of the current best (sqrt is
+// correctly rounded, hence monotone), so sqrt runs only for those.
 template <class real, int DIM>
 __device__ inline bool mirror_culled(real mx, real my, real mz, const Vec<real, DIM>* __restrict__ wpos,
                                      const Vec<real, DIM>* __restrict__ wnrm,
@@ -199,6 +200,7 @@ __device__ inline bool mirror_culled(real mx, real my, real mz, const Vec<real, 
   Cell<real, DIM> q = cell_of<real, DIM>(m, c);
   bool found = false;
   real walldist = real(10) * c.r;
+  real lim = c.r2_hi;
   real wpx = 0, wpy = 0, wpz = 0, wx = 0, wy = 0, wz = 0;
   const int zlo = DIM == 3 ? q.iz - 1 : 0, zhi = DIM == 3 ? q.iz + 1 : 0;
   for (int jz = zlo; jz <= zhi; jz++)
@@ -207,8 +209,11 @@ __device__ inline bool mirror_culled(real mx, real my, real mz, const Vec<real, 
       for (int j = b; j < e; j++) {
         const Vec<real, DIM> w = wpos[j];
         const real jx = w.x, jy2 = w.y, jz2 = vz(w);
-        real dist = sqrt(norm2<real, DIM>(mx - jx, my - jy2, mz - jz2));
+        const real d2 = norm2<real, DIM>(mx - jx, my - jy2, mz - jz2);
+        if (!(d2 <= lim)) continue;
+        real dist = sqrt(d2);
         if (dist <= c.r && dist < walldist) {
+          lim = d2;
           found = true;
           walldist = dist;
           const Vec<real, DIM> n = wnrm[j];
@@ -225,6 +230,252 @@ __device__ inline bool mirror_culled(real mx, real my, real mz, const Vec<real, 
   return found && inout >= real(0);
 }
 
+// Plane mirror of (px,py,pz) across the plane of wall w with normal n
+// (kernel.cu:815-836); false when the particle lies behind the plane.
+template <class real, int DIM>
+__device__ inline bool plane_mirror(real px, real py, real pz, const Vec<real, DIM>& w,
+                                    const Vec<real, DIM>& n, real& mx, real& my, real& mz) {
+  const real wpx = w.x, wpy = w.y, wpz = vz(w);
+  const real nx = n.x, ny = n.y, nz = vz(n);
+  const real PWx = px - wpx, PWy = py - wpy, PWz = pz - wpz;
+  const real PW_WLL = dot3<real, DIM>(PWx, PWy, PWz, nx, ny, nz);
+  if (PW_WLL < real(0)) return false;
+  const real distance = PW_WLL / sqrt(norm2<real, DIM>(nx, ny, nz));
+  mx = wpx + PWx - real(2) * distance * nx;
+  my = wpy + PWy - real(2) * distance * ny;
+  mz = DIM == 3 ? wpz + PWz - real(2) * distance * nz : real(0);
+  return true;
+}
+
+template <class real, int DIM>
+__device__ inline void emit_mirror(real mx, real my, real mz, real prs, int id,
+                                   Vec<real, DIM>* __restrict__ mpos, real* __restrict__ mprs,
+                                   int* __restrict__ mid, int* __restrict__ mcount, int mcap) {
+  const int slot = wave_alloc(mcount);
+  if (slot < mcap) {
+    mpos[slot] = mkvec<real, DIM>(mx, my, mz);
+    mprs[slot] = prs;
+    mid[slot] = id;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Wave-parallel mirror generation (at most 64 wall-normal classes).
+//
+// k_mirror_near lists the live fluid particles whose stencil holds any wall.
+// k_mirror_wave then gives each listed particle one wavefront:
+//  * the lanes split the wall candidates of the stencil; in the reference's
+//    scan order (z, y, x cells, then id) the candidates appear in increasing
+//    wall-array index, so "the first wall met of a normal within r"
+//    (kernel.cu:758-775) is the smallest index of that class within r: an
+//    LDS atomicMin per class.  Slot k of the reference's FromWLL registry is
+//    the class with the k-th smallest first index.  Edge_unique
+//    (kernel.cu:777-795) is the same over Edge walls.
+//  * each plane mirror and the edge mirror is culled by d_MRRinout's
+//    nearest-wall test (kernel.cu:944-983): the lanes split the candidates,
+//    keep the first minimum of their share, and a wave reduction picks the
+//    smallest (dist, index), which is the reference's first-met minimum.
+//  * slot k of listed particle t goes to raw entry 10*t + k (id -1 if empty).
+
+constexpr int kInf = 0x7fffffff;
+
+template <class real, int DIM>
+__global__ __launch_bounds__(kBlock) void k_mirror_near(const Vec<real, DIM>* __restrict__ fpos,
+                                                       const int* __restrict__ fcs, int n_alloc,
+                                                       const int* __restrict__ wcs, Cst<real> c,
+                                                       int* __restrict__ near_list,
+                                                       int* __restrict__ near_count) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_alloc || i >= fcs[c.ncell]) return;
+  const Cell<real, DIM> q = cell_of<real, DIM>(fpos[i], c);
+  bool any = false;
+  const int zlo = DIM == 3 ? q.iz - 1 : 0, zhi = DIM == 3 ? q.iz + 1 : 0;
+  for (int jz = zlo; jz <= zhi; jz++)
+    for (int jy = q.iy - 1; jy <= q.iy + 1; jy++)
+      any = any || wcs[cell_key(q.ix - 1, jy, jz, c)] != wcs[cell_key(q.ix + 1, jy, jz, c) + 1];
+  if (any) near_list[wave_alloc(near_count)] = i;
+}
+
+template <class real>
+struct BestWall {
+  real dist;
+  int j;
+};
+
+// Wave-wide nearest wall within r of m (first met wins ties); j = -1 if none.
+template <class real, int DIM>
+__device__ inline BestWall<real> wave_nearest_wall(real mx, real my, real mz,
+                                                   const Vec<real, DIM>* __restrict__ wpos,
+                                                   const int* __restrict__ wcs, const Cst<real>& c,
+                                                   int lane) {
+  const Cell<real, DIM> q = cell_of<real, DIM>(mkvec<real, DIM>(mx, my, mz), c);
+  real best = real(10) * c.r;
+  real lim = c.r2_hi;
+  int bj = -1;
+  const int zlo = DIM == 3 ? q.iz - 1 : 0, zhi = DIM == 3 ? q.iz + 1 : 0;
+  for (int jz = zlo; jz <= zhi; jz++)
+    for (int jy = q.iy - 1; jy <= q.iy + 1; jy++) {
+      const int b = wcs[cell_key(q.ix - 1, jy, jz, c)], e = wcs[cell_key(q.ix + 1, jy, jz, c) + 1];
+      for (int j = b + lane; j < e; j += 64) {
+        const Vec<real, DIM> w = wpos[j];
+        const real d2 = norm2<real, DIM>(mx - w.x, my - w.y, mz - vz(w));
+        if (!(d2 <= lim)) continue;
+        const real dist = sqrt(d2);
+        if (dist <= c.r && dist < best) {
+          best = dist;
+          bj = j;
+          lim = d2;
+        }
+      }
+    }
+  // lexicographic (dist, j) minimum over the wave; lanes without a hit hold j = -1
+  for (int off = 32; off >= 1; off >>= 1) {
+    const real od = __shfl_xor(best, off);
+    const int oj = __shfl_xor(bj, off);
+    const bool take = oj >= 0 && (bj < 0 || od < best || (od == best && oj < bj));
+    if (take) {
+      best = od;
+      bj = oj;
+    }
+  }
+  return {best, bj};
+}
+
+template <class real, int DIM>
+__global__ __launch_bounds__(64) void k_mirror_wave(
+    const Vec<real, DIM>* __restrict__ fpos, const real* __restrict__ fprs,
+    const int* __restrict__ fid, const int* __restrict__ near_list,
+    const int* __restrict__ near_count, const Vec<real, DIM>* __restrict__ wpos,
+    const Vec<real, DIM>* __restrict__ wnrm, const uint8_t* __restrict__ wcls,
+    const int* __restrict__ wcs, Cst<real> c, Vec<real, DIM>* __restrict__ mpos,
+    real* __restrict__ mprs, int* __restrict__ mid) {
+  __shared__ int first_reg[64], first_edge[64], sorted_reg[kNumMRR], sorted_edge[kNumMRR];
+  __shared__ Vec<real, DIM> s_pos[kNumMRR];
+  __shared__ real s_prs[kNumMRR];
+  __shared__ int s_id[kNumMRR];
+  const int lane = threadIdx.x;
+  const int nnear = *near_count;
+  for (int t = blockIdx.x; t < nnear; t += gridDim.x) {
+    const int i = near_list[t];
+    const Vec<real, DIM> p = fpos[i];
+    const real px = p.x, py = p.y, pz = vz(p);
+    const Cell<real, DIM> q = cell_of<real, DIM>(p, c);
+    first_reg[lane] = kInf;
+    first_edge[lane] = kInf;
+    if (lane < kNumMRR) s_id[lane] = -1;
+    __syncthreads();
+    const real r2_2 = real(1) * c.r2;
+    const int zlo = DIM == 3 ? q.iz - 1 : 0, zhi = DIM == 3 ? q.iz + 1 : 0;
+    for (int jz = zlo; jz <= zhi; jz++)
+      for (int jy = q.iy - 1; jy <= q.iy + 1; jy++) {
+        const int b = wcs[cell_key(q.ix - 1, jy, jz, c)], e = wcs[cell_key(q.ix + 1, jy, jz, c) + 1];
+        for (int j = b + lane; j < e; j += 64) {
+          const Vec<real, DIM> w = wpos[j];
+          if (norm2<real, DIM>(px - w.x, py - w.y, pz - vz(w)) <= r2_2) {
+            const int cl = wcls[j];
+            atomicMin(&first_reg[cl & 63], j);
+            if (cl & 0x80) atomicMin(&first_edge[cl & 63], j);
+          }
+        }
+      }
+    __syncthreads();
+    // registry order: rank of each class's first index
+    const int fr = first_reg[lane], fe = first_edge[lane];
+    int rr = 0, re_ = 0;
+    for (int k = 0; k < 64; k++) {
+      rr += first_reg[k] < fr;
+      re_ += first_edge[k] < fe;
+    }
+    if (fr != kInf && rr < kNumMRR) sorted_reg[rr] = fr;
+    if (fe != kInf && re_ < kNumMRR) sorted_edge[re_] = fe;
+    const int nreg = min(__popcll(__ballot(fr != kInf)), kNumMRR);
+    const int ne = min(__popcll(__ballot(fe != kInf)), kNumMRR);
+    __syncthreads();
+    const int base_id = fid[i] * kNumMRR;
+    const real prs_i = fprs[i];
+    for (int k = 0; k < nreg; k++) {  // plane mirrors, kernel.cu:811-838
+      const int F = sorted_reg[k];
+      real mx, my, mz;
+      if (!plane_mirror<real, DIM>(px, py, pz, wpos[F], wnrm[F], mx, my, mz)) continue;
+      const BestWall<real> bw = wave_nearest_wall<real, DIM>(mx, my, mz, wpos, wcs, c, lane);
+      bool culled = false;
+      if (bw.j >= 0) {
+        const Vec<real, DIM> w = wpos[bw.j], n = wnrm[bw.j];
+        culled = dot3<real, DIM>(mx - w.x, my - w.y, mz - vz(w), n.x, n.y, vz(n)) >= real(0);
+      }
+      if (!culled && lane == 0) {
+        s_pos[k] = mkvec<real, DIM>(mx, my, mz);
+        s_prs[k] = prs_i;
+        s_id[k] = base_id + k;
+      }
+    }
+    if (ne >= 2 && nreg < kNumMRR) {  // edge mirror, kernel.cu:840-888
+      real sx = 0, sy = 0, sz = 0;
+      for (int k = 0; k < ne; k++) {
+        const Vec<real, DIM> n = wnrm[sorted_edge[k]];
+        sx += n.x;
+        sy += n.y;
+        sz += vz(n);
+      }
+      const Vec<real, DIM> w = wpos[sorted_edge[ne - 1]];
+      const real wpx = w.x, wpy = w.y, wpz = vz(w);
+      const real PWx = px - wpx, PWy = py - wpy, PWz = pz - wpz;
+      const real abs_s = sqrt(norm2<real, DIM>(sx, sy, sz));
+      sx /= abs_s;
+      sy /= abs_s;
+      sz /= abs_s;
+      const real PW_WLL = dot3<real, DIM>(PWx, PWy, PWz, sx, sy, sz);
+      if (PW_WLL >= real(0)) {
+        const real distance = PW_WLL / sqrt(norm2<real, DIM>(sx, sy, sz));
+        const real mx = wpx + PWx - real(2) * distance * sx;
+        const real my = wpy + PWy - real(2) * distance * sy;
+        const real mz = DIM == 3 ? wpz + PWz - real(2) * distance * sz : real(0);
+        const BestWall<real> bw = wave_nearest_wall<real, DIM>(mx, my, mz, wpos, wcs, c, lane);
+        bool culled = false;
+        if (bw.j >= 0) {
+          const Vec<real, DIM> w2 = wpos[bw.j], n = wnrm[bw.j];
+          culled = dot3<real, DIM>(mx - w2.x, my - w2.y, mz - vz(w2), n.x, n.y, vz(n)) >= real(0);
+        }
+        if (!culled && lane == 0) {
+          s_pos[nreg] = mkvec<real, DIM>(mx, my, mz);
+          s_prs[nreg] = prs_i;
+          s_id[nreg] = base_id + nreg;
+        }
+      }
+    }
+    __syncthreads();
+    if (lane < kNumMRR) {
+      const size_t o = (size_t)t * kNumMRR + lane;
+      mid[o] = s_id[lane];
+      if (s_id[lane] >= 0) {
+        mpos[o] = s_pos[lane];
+        mprs[o] = s_prs[lane];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Bucket key of a raw mirror slot; empty slots (id < 0) get key -1 and are
+// neither counted nor scattered.
+template <class real, int DIM>
+__global__ void k_mirror_key(const Vec<real, DIM>* __restrict__ pos, const int* __restrict__ id,
+                             const int* __restrict__ n_dyn, int mult, Cst<real> c,
+                             int* __restrict__ key, int* __restrict__ slot, int* __restrict__ cnt) {
+  const int n = *n_dyn * mult;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    if (id[i] < 0) {
+      key[i] = -1;
+      continue;
+    }
+    Cell<real, DIM> q = cell_of<real, DIM>(pos[i], c);
+    const int k = cell_key(q.ix, q.iy, q.iz, c);
+    key[i] = k;
+    slot[i] = atomicAdd(&cnt[k], 1);
+  }
+}
+
+// General mirror generation (any number of distinct wall normals).
 template <class real, int DIM>
 __global__ __launch_bounds__(kBlock) void k_mirror_gen(
     const Vec<real, DIM>* __restrict__ fpos, const real* __restrict__ fprs,
@@ -385,7 +636,9 @@ __global__ __launch_bounds__(kBlock) void k_vsc(
         if (j == i) continue;
         const Vec<real, DIM> pj = pos[j];
         const real qx = pj.x - px, qy = pj.y - py, qz = vz(pj) - pz;
-        const real dist = sqrt(norm2<real, DIM>(qx, qy, qz));
+        const real d2 = norm2<real, DIM>(qx, qy, qz);
+        if (!(d2 <= c.r2_hi)) continue;
+        const real dist = sqrt(d2);
         if (dist <= r) {
           const Vec<real, DIM> vj = vel[j];
           ax += real(2) * (vj.x - vx) * l0 / dist / dist / dist;
@@ -400,7 +653,9 @@ __global__ __launch_bounds__(kBlock) void k_vsc(
       for (int j = b; j < e; j++) {
         const Vec<real, DIM> pj = mpos[j];
         const real qx = pj.x - px, qy = pj.y - py, qz = vz(pj) - pz;
-        const real dist = sqrt(norm2<real, DIM>(qx, qy, qz));
+        const real d2 = norm2<real, DIM>(qx, qy, qz);
+        if (!(d2 <= c.r2_hi)) continue;
+        const real dist = sqrt(d2);
         if (dist <= r) {
           ax += real(2) * (real(0) - vx) * l0 / dist / dist / dist;
           ay += real(2) * (real(0) - vy) * l0 / dist / dist / dist;
@@ -546,7 +801,9 @@ __global__ __launch_bounds__(kBlock) void k_prs(const Vec<real, DIM>* __restrict
       for (int j = b; j < e; j++) {
         if (j == i) continue;
         const Vec<real, DIM> pj = pos[j];
-        const real dist = sqrt(norm2<real, DIM>(pj.x - px, pj.y - py, vz(pj) - pz));
+        const real d2 = norm2<real, DIM>(pj.x - px, pj.y - py, vz(pj) - pz);
+        if (!(d2 <= c.rp2_hi)) continue;
+        const real dist = sqrt(d2);
         if (dist < rp) ni += (rp / dist) - (dist / rp);
       }
     }
@@ -555,7 +812,9 @@ __global__ __launch_bounds__(kBlock) void k_prs(const Vec<real, DIM>* __restrict
       const int b = mcs[cell_key(q.ix - 1, jy, jz, c)], e = mcs[cell_key(q.ix + 1, jy, jz, c) + 1];
       for (int j = b; j < e; j++) {
         const Vec<real, DIM> pj = mpos[j];
-        const real dist = sqrt(norm2<real, DIM>(pj.x - px, pj.y - py, vz(pj) - pz));
+        const real d2 = norm2<real, DIM>(pj.x - px, pj.y - py, vz(pj) - pz);
+        if (!(d2 <= c.rp2_hi)) continue;
+        const real dist = sqrt(d2);
         if (dist < rp) ni += (rp / dist) - (dist / rp);
       }
     }
