@@ -50,6 +50,29 @@ def algorithmic_bytes(kernel: str, nf: int, nw: int, m: int) -> float:
     return float(table.get(kernel, 0))
 
 
+KERNEL_SYMBOL = {  # timing name -> kernel symbol in the rocprofv3 summaries
+    "vsc_up1": "mps::k_vsc<float, 3, true>",
+    "col": "mps::k_col_list<float, 3, false>",
+    "prs": "mps::k_prs_list<float, 3>",
+    "grad_up2": "mps::k_grad_list<float, 3, true>",
+    "mirror_gen": "mps::k_mirror_wave<float, 3>",
+    "nlist": "mps::k_nlist_build<float, 3>",
+}
+
+
+def measured_traffic(kernel: str):
+    """HBM-side bytes per launch of `kernel` from the committed FETCH_SIZE /
+    WRITE_SIZE summary (copy-calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", "cfg1_hbm_traffic.md")
+    sym = KERNEL_SYMBOL.get(kernel)
+    if not sym or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith(f"| `{sym}` |"):
+            return float(line.strip().strip("|").split("|")[-1])
+    return None
+
+
 def candidates_per_pass(h) -> int:
     """Fluid candidates the 27-cell stencil visits, summed over live fluid
     particles (one pass of a fluid neighbour loop)."""
@@ -90,7 +113,18 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (calibrates HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        a = torch.empty(args.copy_probe << 18, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_each_way": a.numel() * 4, "launches": 3}))
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: this build has no multi-GPU path (asked for {args.gpus})",
               file=sys.stderr)
@@ -130,8 +164,10 @@ def main():
     passes = {"vsc_up1": 1, "prs": 1, "grad_up2": 2, "col": 1}.get(dom, 1)
     flops = FLOP_PER_CANDIDATE * candidates_per_pass(h) * passes
     tflops = flops / (avg_ms * 1e-3) / 1e12
+    traffic = measured_traffic(dom)
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": round(traffic) if traffic else None,
                 "bytes_per_launch": bytes_, "avg_ms": round(avg_ms, 5),
                 "valu": {"achieved": round(tflops, 2), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                          "frac": round(tflops / FP32_PEAK_TFLOPS, 4), "flops_per_launch": flops}}

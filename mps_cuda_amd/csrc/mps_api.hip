@@ -4,6 +4,7 @@
 // all work runs on one HIP stream with no host synchronisation inside
 // mps_step (the reference synchronises after every phase, 18 times a step:
 // kernel.cu:544-1841).
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -17,6 +18,7 @@
 #include "../../include/mps.h"
 #include "mps_host.h"
 #include "mps_kernels.h"
+#include "mps_nlist.h"
 
 namespace {
 
@@ -97,8 +99,7 @@ struct ImplBase {
     pool.pop_back();
     return e;
   }
-  void tick(const char* nm) {
-    if (!timing) return;
+  Rec& open_rec(const char* nm) {
     int id = -1;
     for (size_t k = 0; k < tnames.size(); k++)
       if (tnames[k] == nm) id = (int)k;
@@ -108,14 +109,21 @@ struct ImplBase {
       tms.push_back(0.0);
       tcount.push_back(0);
     }
-    Rec r{id, event(), event()};
-    (void)hipEventRecord(r.a, stream);
-    recs.push_back(r);
+    recs.push_back(Rec{id, event(), event()});
+    return recs.back();
+  }
+  void close_rec() {
+    if (recs.size() >= 8192) (void)flush_timing();
+  }
+  // a sequence of launches timed as one (events recorded on the stream)
+  void tick(const char* nm) {
+    if (!timing) return;
+    (void)hipEventRecord(open_rec(nm).a, stream);
   }
   void tock() {
     if (!timing || recs.empty()) return;
     (void)hipEventRecord(recs.back().b, stream);
-    if (recs.size() >= 8192) (void)flush_timing();
+    close_rec();
   }
   int flush_timing() {
     if (recs.empty()) return MPS_OK;
@@ -133,11 +141,18 @@ struct ImplBase {
   }
 };
 
-#define TLAUNCH(name, ...)          \
-  do {                              \
-    tick(name);                     \
-    hipLaunchKernelGGL(__VA_ARGS__); \
-    tock();                         \
+// Launch; while timing, the start/stop events ride in the dispatch packet
+// (hipExtLaunchKernelGGL), so they time the kernel itself and not the
+// host's enqueue gaps.
+#define TLAUNCH(name, kernel, grid, block, shm, strm, ...)                                    \
+  do {                                                                                        \
+    if (timing) {                                                                             \
+      Rec& r_ = open_rec(name);                                                               \
+      hipExtLaunchKernelGGL(kernel, grid, block, shm, strm, r_.a, r_.b, 0, __VA_ARGS__);      \
+      close_rec();                                                                            \
+    } else {                                                                                  \
+      hipLaunchKernelGGL(kernel, grid, block, shm, strm, __VA_ARGS__);                        \
+    }                                                                                         \
   } while (0)
 
 template <class real, int DIM>
@@ -169,6 +184,8 @@ struct Impl : ImplBase {
   real *mraw_prs = nullptr, *mprs = nullptr;
   int *mraw_id = nullptr, *mid = nullptr, *mcs = nullptr, *mcount = nullptr;
   int* near_list = nullptr;  // fluid particles with walls in their stencil
+  int *nlist = nullptr, *ncount = nullptr;  // ordered fluid neighbours within rp
+  uint8_t *wnear = nullptr, *mnear = nullptr;  // any wall / mirror in a bucket's stencil
   // bucket scratch (shared by the fluid and mirror builds)
   int *key = nullptr, *slot = nullptr, *tmp = nullptr, *perm = nullptr, *cnt = nullptr;
   // scan levels: parts[l] holds block totals, parts_ex[l] their exclusive scan
@@ -261,7 +278,9 @@ struct Impl : ImplBase {
     if ((rc = alloc(&mraw_pos, mcap)) || (rc = alloc(&mpos, mcap)) || (rc = alloc(&mraw_prs, mcap)) ||
         (rc = alloc(&mprs, mcap)) || (rc = alloc(&mraw_id, mcap)) || (rc = alloc(&mid, mcap)) ||
         (rc = alloc(&mcs, (size_t)ncell + 2)) || (rc = alloc(&mcount, 1)) ||
-        (rc = alloc(&near_list, std::max(nf, 1))))
+        (rc = alloc(&near_list, std::max(nf, 1))) || (rc = alloc(&nlist, (size_t)mps::kNList * std::max(nf, 1))) ||
+        (rc = alloc(&ncount, std::max(nf, 1))) || (rc = alloc(&wnear, std::max(ncell, 1))) ||
+        (rc = alloc(&mnear, std::max(ncell, 1))))
       return rc;
     const size_t scr = std::max(mcap, nf);
     if ((rc = alloc(&key, scr)) || (rc = alloc(&slot, scr)) || (rc = alloc(&tmp, scr)) ||
@@ -322,6 +341,8 @@ struct Impl : ImplBase {
         hcls[w] = uint8_t(k & 63) | (he[w] == MPS_EDGE ? 0x80 : 0);
       }
       HIP_TRY(hipMemcpy(wcls, hcls.data(), nw, hipMemcpyHostToDevice));
+      hipLaunchKernelGGL((mps::k_cell_near<real, DIM>), dim3(nblk(ncell)), dim3(mps::kBlock), 0, stream,
+                         (const int*)wcs, c, wnear);
     }
     // upload the fluid set in creation order, then bucket it (ClcMPS starts
     // with MkBkt, kernel.cu:1878)
@@ -354,6 +375,8 @@ struct Impl : ImplBase {
     hipLaunchKernelGGL(mps::k_fill_int, dim3(nblk(ncell + 2, 4096)), dim3(mps::kBlock), 0, stream,
                        mcs, ncell + 2, 0);
     hipLaunchKernelGGL(mps::k_fill_int, dim3(1), dim3(mps::kBlock), 0, stream, mcount, 1, 0);
+    hipLaunchKernelGGL((mps::k_fill<uint8_t>), dim3(nblk(ncell, 4096)), dim3(mps::kBlock), 0, stream,
+                       mnear, ncell, (uint8_t)0);
     if ((rc = bucket())) return rc;
     HIP_TRY(hipStreamSynchronize(stream));
     return MPS_OK;
@@ -419,7 +442,7 @@ struct Impl : ImplBase {
     int mult = 1;
     if (nf > 0 && use_cls) {
       TLAUNCH("mirror_near", (mps::k_mirror_near<real, DIM>), dim3(nblk(nf)), B, 0, stream, fpos[0],
-              fcs, nf, wcs, c, near_list, mcount);
+              fcs, nf, (const uint8_t*)wnear, c, near_list, mcount);
       TLAUNCH("mirror_gen", (mps::k_mirror_wave<real, DIM>), dim3(std::min(nf, 16384)), dim3(64), 0,
               stream, fpos[0], fprs[0], fid[0], near_list, mcount, wpos, wnrm, wcls, wcs, c,
               mraw_pos, mraw_prs, mraw_id);
@@ -443,6 +466,8 @@ struct Impl : ImplBase {
     // live mirrors: mcs[ncell] (empty slots sit in the tail cell)
     TLAUNCH("mirror_gather", (mps::k_gather_mirror<real, DIM>), dim3(g), B, 0, stream, perm,
             (const int*)(mcs + ncell), mraw_pos, mraw_prs, mraw_id, mpos, mprs, mid);
+    TLAUNCH("mirror_flags", (mps::k_cell_near<real, DIM>), dim3(nblk(ncell)), B, 0, stream,
+            (const int*)mcs, c, mnear);
     return MPS_OK;
   }
 
@@ -452,11 +477,71 @@ struct Impl : ImplBase {
     return MPS_OK;
   }
 
+  // ---- phase launchers ------------------------------------------------------
+  // d_VscTrm (+ d_UpPcl1 when fuse): fuse writes the new pos/vel and swaps
+  // them in; otherwise the acceleration goes to facc.
+  void run_vsc(bool fuse) {
+    const dim3 B(mps::kBlock), G(nblk(nf));
+    if (fuse) {
+      TLAUNCH("vsc_up1", (mps::k_vsc<real, DIM, true>), G, B, 0, stream, fpos[0], fvel[0], fcs, nf,
+              mpos, mcs, (const uint8_t*)mnear, c, (V*)nullptr, fpos[1], fvel[1], ftyp[0], fprs[0]);
+      std::swap(fpos[0], fpos[1]);
+      std::swap(fvel[0], fvel[1]);
+    } else {
+      TLAUNCH("vsc", (mps::k_vsc<real, DIM, false>), G, B, 0, stream, fpos[0], fvel[0], fcs, nf, mpos,
+              mcs, (const uint8_t*)mnear, c, facc[0], (V*)nullptr, (V*)nullptr, (int8_t*)nullptr, (real*)nullptr);
+    }
+  }
+
+  // Ordered fluid neighbour lists within rp for ChkCol/MkPrs/PrsGrdTrm.
+  void run_nlist() {
+    TLAUNCH("nlist", (mps::k_nlist_build<real, DIM>), dim3(nblk(nf)), dim3(mps::kBlock), 0, stream,
+            fpos[0], fcs, nf, c, nlist, ncount);
+  }
+
+  // d_ChkCol + d_UpChkCol: new velocities into fvel[1], swapped in.
+  void run_col(bool phase_mode) {
+    const dim3 B(mps::kBlock), G(nblk(nf));
+    if (phase_mode)
+      TLAUNCH("col", (mps::k_col_list<real, DIM, true>), G, B, 0, stream, fpos[0], fvel[0], fcs, nf,
+              nlist, ncount, wpos, wvel, wcs, mpos, mcs, (const uint8_t*)wnear,
+              (const uint8_t*)mnear, c, fvel[1], facc[0]);
+    else
+      TLAUNCH("col", (mps::k_col_list<real, DIM, false>), G, B, 0, stream, fpos[0], fvel[0], fcs, nf,
+              nlist, ncount, wpos, wvel, wcs, mpos, mcs, (const uint8_t*)wnear,
+              (const uint8_t*)mnear, c, fvel[1], (V*)nullptr);
+    std::swap(fvel[0], fvel[1]);
+  }
+
+  void run_prs() {
+    TLAUNCH("prs", (mps::k_prs_list<real, DIM>), dim3(nblk(nf)), dim3(mps::kBlock), 0, stream, fpos[0],
+            fcs, nf, nlist, ncount, mpos, mcs, (const uint8_t*)mnear, c, fprs[0], fnd[0]);
+  }
+
+  // d_PrsGrdTrm (+ d_UpPcl2 when fuse: new pos/vel/prs swapped in, pav += prs).
+  void run_grad(bool fuse) {
+    const dim3 B(mps::kBlock), G(nblk(nf));
+    if (fuse) {
+      TLAUNCH("grad_up2", (mps::k_grad_list<real, DIM, true>), G, B, 0, stream, fpos[0], fvel[0],
+              fprs[0], fcs, nf, nlist, ncount, mpos, mprs, mcs, (const uint8_t*)mnear, c, (V*)nullptr,
+              fpos[1], fvel[1],
+              fprs[1], fpav[0], ftyp[0]);
+      std::swap(fpos[0], fpos[1]);
+      std::swap(fvel[0], fvel[1]);
+      std::swap(fprs[0], fprs[1]);
+    } else {
+      TLAUNCH("grad", (mps::k_grad_list<real, DIM, false>), G, B, 0, stream, fpos[0], fvel[0], fprs[0],
+              fcs, nf, nlist, ncount, mpos, mprs, mcs, (const uint8_t*)mnear, c, facc[0],
+              (V*)nullptr, (V*)nullptr,
+              (real*)nullptr, (real*)nullptr, (int8_t*)nullptr);
+    }
+  }
+
   // One ClcMPS loop body (kernel.cu:1904-1935) with fused kernels.  The
   // redundant MkBkt of kernel.cu:1921 is skipped: positions have not
   // changed since kernel.cu:1917, so it would rebuild identical buckets.
   int step(int nsteps) override {
-    const dim3 B(mps::kBlock), G(nblk(nf));
+    const dim3 B(mps::kBlock);
     if (acc_live) {
       hipLaunchKernelGGL((mps::k_fill<V>), dim3(nblk(nf, 4096)), B, 0, stream, facc[0], nf,
                          mps::mkvec<real, DIM>(0, 0, 0));
@@ -465,29 +550,15 @@ struct Impl : ImplBase {
     for (int s = 0; s < nsteps; s++) {
       bucket();
       mirror();
-      if (nf > 0) {
-        TLAUNCH("vsc_up1", (mps::k_vsc<real, DIM, true>), G, B, 0, stream, fpos[0], fvel[0], fcs, nf,
-                           mpos, mcs, c, (V*)nullptr, fpos[1], fvel[1], ftyp[0], fprs[0]);
-        std::swap(fpos[0], fpos[1]);
-        std::swap(fvel[0], fvel[1]);
-      }
+      if (nf > 0) run_vsc(true);
       bucket();
       if (nf > 0) {
-        TLAUNCH("col", (mps::k_col<real, DIM, false>), G, B, 0, stream, fpos[0], fvel[0], fcs, nf,
-                           wpos, wvel, wcs, mpos, mcs, c, fvel[1], (V*)nullptr);
-        std::swap(fvel[0], fvel[1]);
-        TLAUNCH("prs", (mps::k_prs<real, DIM>), G, B, 0, stream, fpos[0], fcs, nf, mpos, mcs, c,
-                           fprs[0], fnd[0]);
+        run_nlist();
+        run_col(false);
+        run_prs();
       }
       mirror();
-      if (nf > 0) {
-        TLAUNCH("grad_up2", (mps::k_grad<real, DIM, true>), G, B, 0, stream, fpos[0], fvel[0], fprs[0],
-                           fcs, nf, mpos, mprs, mcs, c, (V*)nullptr, fpos[1], fvel[1], fprs[1],
-                           fpav[0], ftyp[0]);
-        std::swap(fpos[0], fpos[1]);
-        std::swap(fvel[0], fvel[1]);
-        std::swap(fprs[0], fprs[1]);
-      }
+      if (nf > 0) run_grad(true);
       if (wall_pav && nw > 0)
         hipLaunchKernelGGL((mps::k_wall_pav<real>), dim3(nblk(nw)), B, 0, stream, wpav, wprs, nw);
     }
@@ -501,10 +572,7 @@ struct Impl : ImplBase {
       case MPS_PH_MKBKT: bucket(); break;
       case MPS_PH_MIRROR: mirror(); break;
       case MPS_PH_VSC:
-        if (nf > 0)
-          hipLaunchKernelGGL((mps::k_vsc<real, DIM, false>), G, B, 0, stream, fpos[0], fvel[0], fcs, nf,
-                             mpos, mcs, c, facc[0], (V*)nullptr, (V*)nullptr, (int8_t*)nullptr,
-                             (real*)nullptr);
+        if (nf > 0) run_vsc(false);
         break;
       case MPS_PH_UP1:
         if (nf > 0)
@@ -513,21 +581,21 @@ struct Impl : ImplBase {
         break;
       case MPS_PH_COL:
         if (nf > 0) {
-          hipLaunchKernelGGL((mps::k_col<real, DIM, true>), G, B, 0, stream, fpos[0], fvel[0], fcs, nf,
-                             wpos, wvel, wcs, mpos, mcs, c, fvel[1], facc[0]);
-          std::swap(fvel[0], fvel[1]);
+          run_nlist();
+          run_col(true);
         }
         break;
       case MPS_PH_PRS:
-        if (nf > 0)
-          hipLaunchKernelGGL((mps::k_prs<real, DIM>), G, B, 0, stream, fpos[0], fcs, nf, mpos, mcs, c,
-                             fprs[0], fnd[0]);
+        if (nf > 0) {
+          run_nlist();
+          run_prs();
+        }
         break;
       case MPS_PH_GRAD:
-        if (nf > 0)
-          hipLaunchKernelGGL((mps::k_grad<real, DIM, false>), G, B, 0, stream, fpos[0], fvel[0],
-                             fprs[0], fcs, nf, mpos, mprs, mcs, c, facc[0], (V*)nullptr, (V*)nullptr,
-                             (real*)nullptr, (real*)nullptr, (int8_t*)nullptr);
+        if (nf > 0) {
+          run_nlist();
+          run_grad(false);
+        }
         break;
       case MPS_PH_UP2:
         if (nf > 0)

@@ -86,6 +86,30 @@ __global__ __launch_bounds__(kBlock) void k_scan_down(const int* in, int n, cons
 }
 
 // ---------------------------------------------------------------------------
+// near[k] = 1 when any entry of the set with cell_start cs lies in the
+// 27-bucket (9 in 2-D) stencil of bucket k.  Lets a particle skip a stencil
+// pass over a set that is empty around it.
+template <class real, int DIM>
+__global__ void k_cell_near(const int* __restrict__ cs, Cst<real> c, uint8_t* __restrict__ near) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= c.ncell) return;
+  const int iz = DIM == 3 ? k / c.nbxy : 0;
+  const int rem = k - iz * c.nbxy;
+  const int iy = rem / c.nb[0], ix = rem - iy * c.nb[0];
+  if (ix < 1 || ix > c.nb[0] - 2 || iy < 1 || iy > c.nb[1] - 2 ||
+      (DIM == 3 && (iz < 1 || iz > c.nb[2] - 2))) {
+    near[k] = 0;  // no particle is ever assigned to a border bucket
+    return;
+  }
+  bool any = false;
+  const int zlo = DIM == 3 ? iz - 1 : 0, zhi = DIM == 3 ? iz + 1 : 0;
+  for (int jz = zlo; jz <= zhi; jz++)
+    for (int jy = iy - 1; jy <= iy + 1; jy++)
+      any = any || cs[cell_key(ix - 1, jy, jz, c)] != cs[cell_key(ix + 1, jy, jz, c) + 1];
+  near[k] = any ? 1 : 0;
+}
+
+// ---------------------------------------------------------------------------
 // Bucket build: key + count, scatter, rank inside the cell by id, gather.
 
 template <class real, int DIM>
@@ -282,18 +306,13 @@ constexpr int kInf = 0x7fffffff;
 template <class real, int DIM>
 __global__ __launch_bounds__(kBlock) void k_mirror_near(const Vec<real, DIM>* __restrict__ fpos,
                                                        const int* __restrict__ fcs, int n_alloc,
-                                                       const int* __restrict__ wcs, Cst<real> c,
+                                                       const uint8_t* __restrict__ wnear, Cst<real> c,
                                                        int* __restrict__ near_list,
                                                        int* __restrict__ near_count) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n_alloc || i >= fcs[c.ncell]) return;
   const Cell<real, DIM> q = cell_of<real, DIM>(fpos[i], c);
-  bool any = false;
-  const int zlo = DIM == 3 ? q.iz - 1 : 0, zhi = DIM == 3 ? q.iz + 1 : 0;
-  for (int jz = zlo; jz <= zhi; jz++)
-    for (int jy = q.iy - 1; jy <= q.iy + 1; jy++)
-      any = any || wcs[cell_key(q.ix - 1, jy, jz, c)] != wcs[cell_key(q.ix + 1, jy, jz, c) + 1];
-  if (any) near_list[wave_alloc(near_count)] = i;
+  if (wnear[cell_key(q.ix, q.iy, q.iz, c)]) near_list[wave_alloc(near_count)] = i;
 }
 
 template <class real>
@@ -603,15 +622,37 @@ __device__ inline bool chk_pcl(real& x, real& y, real& z, real& vx, real& vy, re
 }
 
 // ---------------------------------------------------------------------------
+// Visits, in ascending order, the candidates j of [b, e) that pass test(j)
+// and calls body(j) on each.  The cheap test runs over a chunk of 64
+// candidates first and the hits are kept in a bit mask; the expensive body
+// then runs over the hits only.  Without this, the lanes of a wavefront
+// (each with its own hits) would together take the expensive branch for
+// almost every candidate.  The visiting order is unchanged.
+template <class T, class Body>
+__device__ inline void for_hits(int b, int e, T&& test, Body&& body) {
+  for (int base = b; base < e; base += 64) {
+    const int n = min(64, e - base);
+    unsigned long long mask = 0;
+    for (int t = 0; t < n; t++)
+      if (test(base + t)) mask |= 1ull << t;
+    while (mask) {
+      const int t = __ffsll((long long)mask) - 1;
+      mask &= mask - 1;
+      body(base + t);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // d_VscTrm (kernel.cu:1013-1119), optionally fused with d_UpPcl1
 // (kernel.cu:1138-1149).
 template <class real, int DIM, bool FUSE>
 __global__ __launch_bounds__(kBlock) void k_vsc(
     const Vec<real, DIM>* __restrict__ pos, const Vec<real, DIM>* __restrict__ vel,
     const int* __restrict__ fcs, int n_alloc, const Vec<real, DIM>* __restrict__ mpos,
-    const int* __restrict__ mcs, Cst<real> c, Vec<real, DIM>* __restrict__ acc_out,
-    Vec<real, DIM>* __restrict__ pos_out, Vec<real, DIM>* __restrict__ vel_out,
-    int8_t* __restrict__ typ, real* __restrict__ prs) {
+    const int* __restrict__ mcs, const uint8_t* __restrict__ mnear, Cst<real> c,
+    Vec<real, DIM>* __restrict__ acc_out, Vec<real, DIM>* __restrict__ pos_out,
+    Vec<real, DIM>* __restrict__ vel_out, int8_t* __restrict__ typ, real* __restrict__ prs) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n_alloc) return;
   const int n_live = fcs[c.ncell];
@@ -629,39 +670,48 @@ __global__ __launch_bounds__(kBlock) void k_vsc(
   const Cell<real, DIM> q = cell_of<real, DIM>(p, c);
   const int zlo = DIM == 3 ? q.iz - 1 : 0, zhi = DIM == 3 ? q.iz + 1 : 0;
   const real l0 = c.l0, r = c.r;
+  const real r2_hi = c.r2_hi;
   for (int jz = zlo; jz <= zhi; jz++)
     for (int jy = q.iy - 1; jy <= q.iy + 1; jy++) {
       const int b = fcs[cell_key(q.ix - 1, jy, jz, c)], e = fcs[cell_key(q.ix + 1, jy, jz, c) + 1];
-      for (int j = b; j < e; j++) {
-        if (j == i) continue;
-        const Vec<real, DIM> pj = pos[j];
-        const real qx = pj.x - px, qy = pj.y - py, qz = vz(pj) - pz;
-        const real d2 = norm2<real, DIM>(qx, qy, qz);
-        if (!(d2 <= c.r2_hi)) continue;
-        const real dist = sqrt(d2);
-        if (dist <= r) {
-          const Vec<real, DIM> vj = vel[j];
-          ax += real(2) * (vj.x - vx) * l0 / dist / dist / dist;
-          ay += real(2) * (vj.y - vy) * l0 / dist / dist / dist;
-          if constexpr (DIM == 3) az += real(2) * (vj.z - vvz) * l0 / dist / dist / dist;
-        }
-      }
+      for_hits(
+          b, e,
+          [&](int j) {
+            const Vec<real, DIM> pj = pos[j];
+            return j != i && norm2<real, DIM>(pj.x - px, pj.y - py, vz(pj) - pz) <= r2_hi;
+          },
+          [&](int j) {
+            const Vec<real, DIM> pj = pos[j];
+            const real qx = pj.x - px, qy = pj.y - py, qz = vz(pj) - pz;
+            const real dist = sqrt(norm2<real, DIM>(qx, qy, qz));
+            if (dist <= r) {
+              const Vec<real, DIM> vj = vel[j];
+              ax += real(2) * (vj.x - vx) * l0 / dist / dist / dist;
+              ay += real(2) * (vj.y - vy) * l0 / dist / dist / dist;
+              if constexpr (DIM == 3) az += real(2) * (vj.z - vvz) * l0 / dist / dist / dist;
+            }
+          });
     }
+  if (mnear[cell_key(q.ix, q.iy, q.iz, c)])
   for (int jz = zlo; jz <= zhi; jz++)
     for (int jy = q.iy - 1; jy <= q.iy + 1; jy++) {
       const int b = mcs[cell_key(q.ix - 1, jy, jz, c)], e = mcs[cell_key(q.ix + 1, jy, jz, c) + 1];
-      for (int j = b; j < e; j++) {
-        const Vec<real, DIM> pj = mpos[j];
-        const real qx = pj.x - px, qy = pj.y - py, qz = vz(pj) - pz;
-        const real d2 = norm2<real, DIM>(qx, qy, qz);
-        if (!(d2 <= c.r2_hi)) continue;
-        const real dist = sqrt(d2);
-        if (dist <= r) {
-          ax += real(2) * (real(0) - vx) * l0 / dist / dist / dist;
-          ay += real(2) * (real(0) - vy) * l0 / dist / dist / dist;
-          if constexpr (DIM == 3) az += real(2) * (real(0) - vvz) * l0 / dist / dist / dist;
-        }
-      }
+      for_hits(
+          b, e,
+          [&](int j) {
+            const Vec<real, DIM> pj = mpos[j];
+            return norm2<real, DIM>(pj.x - px, pj.y - py, vz(pj) - pz) <= r2_hi;
+          },
+          [&](int j) {
+            const Vec<real, DIM> pj = mpos[j];
+            const real qx = pj.x - px, qy = pj.y - py, qz = vz(pj) - pz;
+            const real dist = sqrt(norm2<real, DIM>(qx, qy, qz));
+            if (dist <= r) {
+              ax += real(2) * (real(0) - vx) * l0 / dist / dist / dist;
+              ay += real(2) * (real(0) - vy) * l0 / dist / dist / dist;
+              if constexpr (DIM == 3) az += real(2) * (real(0) - vvz) * l0 / dist / dist / dist;
+            }
+          });
     }
   const real accx = c.n0knm * ax + c.g[0];
   const real accy = c.n0knm * ay + c.g[1];
@@ -734,43 +784,51 @@ __global__ __launch_bounds__(kBlock) void k_col(
           const Vec<real, DIM>* __restrict__ P = pass == 0 ? pos : wpos;
           const Vec<real, DIM>* __restrict__ V = pass == 0 ? vel : wvel;
           const real lim2 = pass == 0 ? c.rlim2 : c.rlim_wall2;
-          for (int j = b; j < e; j++) {
-            if (pass == 0 && j == i) continue;
-            const Vec<real, DIM> pj = P[j];
-            const real qx = pj.x - px, qy = pj.y - py, qz = vz(pj) - pz;
-            const real dist2 = norm2<real, DIM>(qx, qy, qz);
-            if (dist2 < lim2) {
-              const Vec<real, DIM> vj = V[j];
-              real fDT = dot3<real, DIM>(vx - vj.x, vy - vj.y, vvz - vz(vj), qx, qy, qz);
-              if (fDT > real(0)) {
-                const real mj = c.mass;
-                fDT *= c.col * mj / (mi + mj) / dist2;
-                v2x -= qx * fDT;
-                v2y -= qy * fDT;
-                if constexpr (DIM == 3) v2z -= qz * fDT;
-              }
-            }
-          }
+          const int self = pass == 0 ? i : -1;
+          for_hits(
+              b, e,
+              [&](int j) {
+                const Vec<real, DIM> pj = P[j];
+                return j != self && norm2<real, DIM>(pj.x - px, pj.y - py, vz(pj) - pz) < lim2;
+              },
+              [&](int j) {
+                const Vec<real, DIM> pj = P[j];
+                const real qx = pj.x - px, qy = pj.y - py, qz = vz(pj) - pz;
+                const real dist2 = norm2<real, DIM>(qx, qy, qz);
+                const Vec<real, DIM> vj = V[j];
+                real fDT = dot3<real, DIM>(vx - vj.x, vy - vj.y, vvz - vz(vj), qx, qy, qz);
+                if (fDT > real(0)) {
+                  const real mj = c.mass;
+                  fDT *= c.col * mj / (mi + mj) / dist2;
+                  v2x -= qx * fDT;
+                  v2y -= qy * fDT;
+                  if constexpr (DIM == 3) v2z -= qz * fDT;
+                }
+              });
         }
       }
   const real mj = c.mass;
   for (int jz = zlo; jz <= zhi; jz++)
     for (int jy = q.iy - 1; jy <= q.iy + 1; jy++) {
       const int b = mcs[cell_key(q.ix - 1, jy, jz, c)], e = mcs[cell_key(q.ix + 1, jy, jz, c) + 1];
-      for (int j = b; j < e; j++) {
-        const Vec<real, DIM> pj = mpos[j];
-        const real qx = pj.x - px, qy = pj.y - py, qz = vz(pj) - pz;
-        const real dist2 = norm2<real, DIM>(qx, qy, qz);
-        if (dist2 < c.rlim2) {
-          real fDT = dot3<real, DIM>(vx - real(0), vy - real(0), vvz - real(0), qx, qy, qz);
-          if (fDT > real(0)) {
-            fDT *= c.col * mj / (mi + mj) / dist2;
-            v2x -= qx * fDT;
-            v2y -= qy * fDT;
-            if constexpr (DIM == 3) v2z -= qz * fDT;
-          }
-        }
-      }
+      for_hits(
+          b, e,
+          [&](int j) {
+            const Vec<real, DIM> pj = mpos[j];
+            return norm2<real, DIM>(pj.x - px, pj.y - py, vz(pj) - pz) < c.rlim2;
+          },
+          [&](int j) {
+            const Vec<real, DIM> pj = mpos[j];
+            const real qx = pj.x - px, qy = pj.y - py, qz = vz(pj) - pz;
+            const real dist2 = norm2<real, DIM>(qx, qy, qz);
+            real fDT = dot3<real, DIM>(vx - real(0), vy - real(0), vvz - real(0), qx, qy, qz);
+            if (fDT > real(0)) {
+              fDT *= c.col * mj / (mi + mj) / dist2;
+              v2x -= qx * fDT;
+              v2y -= qy * fDT;
+              if constexpr (DIM == 3) v2z -= qz * fDT;
+            }
+          });
     }
   const Vec<real, DIM> out = mkvec<real, DIM>(v2x, v2y, v2z);
   vel_out[i] = out;
@@ -798,25 +856,32 @@ __global__ __launch_bounds__(kBlock) void k_prs(const Vec<real, DIM>* __restrict
   for (int jz = zlo; jz <= zhi; jz++)
     for (int jy = q.iy - 1; jy <= q.iy + 1; jy++) {
       const int b = fcs[cell_key(q.ix - 1, jy, jz, c)], e = fcs[cell_key(q.ix + 1, jy, jz, c) + 1];
-      for (int j = b; j < e; j++) {
-        if (j == i) continue;
-        const Vec<real, DIM> pj = pos[j];
-        const real d2 = norm2<real, DIM>(pj.x - px, pj.y - py, vz(pj) - pz);
-        if (!(d2 <= c.rp2_hi)) continue;
-        const real dist = sqrt(d2);
-        if (dist < rp) ni += (rp / dist) - (dist / rp);
-      }
+      for_hits(
+          b, e,
+          [&](int j) {
+            const Vec<real, DIM> pj = pos[j];
+            return j != i && norm2<real, DIM>(pj.x - px, pj.y - py, vz(pj) - pz) <= c.rp2_hi;
+          },
+          [&](int j) {
+            const Vec<real, DIM> pj = pos[j];
+            const real dist = sqrt(norm2<real, DIM>(pj.x - px, pj.y - py, vz(pj) - pz));
+            if (dist < rp) ni += (rp / dist) - (dist / rp);
+          });
     }
   for (int jz = zlo; jz <= zhi; jz++)
     for (int jy = q.iy - 1; jy <= q.iy + 1; jy++) {
       const int b = mcs[cell_key(q.ix - 1, jy, jz, c)], e = mcs[cell_key(q.ix + 1, jy, jz, c) + 1];
-      for (int j = b; j < e; j++) {
-        const Vec<real, DIM> pj = mpos[j];
-        const real d2 = norm2<real, DIM>(pj.x - px, pj.y - py, vz(pj) - pz);
-        if (!(d2 <= c.rp2_hi)) continue;
-        const real dist = sqrt(d2);
-        if (dist < rp) ni += (rp / dist) - (dist / rp);
-      }
+      for_hits(
+          b, e,
+          [&](int j) {
+            const Vec<real, DIM> pj = mpos[j];
+            return norm2<real, DIM>(pj.x - px, pj.y - py, vz(pj) - pz) <= c.rp2_hi;
+          },
+          [&](int j) {
+            const Vec<real, DIM> pj = mpos[j];
+            const real dist = sqrt(norm2<real, DIM>(pj.x - px, pj.y - py, vz(pj) - pz));
+            if (dist < rp) ni += (rp / dist) - (dist / rp);
+          });
     }
   real pressure = real(ni > c.n0_grad) * (ni - c.n0_grad) * c.prs_coef * c.mass;
   if (pressure <= c.pmax && pressure >= real(0)) {
@@ -876,37 +941,44 @@ __global__ __launch_bounds__(kBlock) void k_grad(
   for (int jz = zlo; jz <= zhi; jz++)
     for (int jy = q.iy - 1; jy <= q.iy + 1; jy++) {
       const int b = fcs[cell_key(q.ix - 1, jy, jz, c)], e = fcs[cell_key(q.ix + 1, jy, jz, c) + 1];
-      for (int j = b; j < e; j++) {
-        if (j == i) continue;
-        const Vec<real, DIM> pj = pos[j];
-        const real qx = pj.x - px, qy = pj.y - py, qz = vz(pj) - pz;
-        const real dist2 = norm2<real, DIM>(qx, qy, qz);
-        if (dist2 < rp2) {
-          const real dist = sqrt(dist2);
-          const real w = ((rp / dist) - (dist / rp)) / dist2;
-          const real d = prs[j] - Prs_min;
-          ax += d * w * qx;
-          ay += d * w * qy;
-          if constexpr (DIM == 3) az += d * w * qz;
-        }
-      }
+      for_hits(
+          b, e,
+          [&](int j) {
+            const Vec<real, DIM> pj = pos[j];
+            return j != i && norm2<real, DIM>(pj.x - px, pj.y - py, vz(pj) - pz) < rp2;
+          },
+          [&](int j) {
+            const Vec<real, DIM> pj = pos[j];
+            const real qx = pj.x - px, qy = pj.y - py, qz = vz(pj) - pz;
+            const real dist2 = norm2<real, DIM>(qx, qy, qz);
+            const real dist = sqrt(dist2);
+            const real w = ((rp / dist) - (dist / rp)) / dist2;
+            const real d = prs[j] - Prs_min;
+            ax += d * w * qx;
+            ay += d * w * qy;
+            if constexpr (DIM == 3) az += d * w * qz;
+          });
     }
   for (int jz = zlo; jz <= zhi; jz++)
     for (int jy = q.iy - 1; jy <= q.iy + 1; jy++) {
       const int b = mcs[cell_key(q.ix - 1, jy, jz, c)], e = mcs[cell_key(q.ix + 1, jy, jz, c) + 1];
-      for (int j = b; j < e; j++) {
-        const Vec<real, DIM> pj = mpos[j];
-        const real qx = pj.x - px, qy = pj.y - py, qz = vz(pj) - pz;
-        const real dist2 = norm2<real, DIM>(qx, qy, qz);
-        if (dist2 < rp2) {
-          const real dist = sqrt(dist2);
-          const real w = ((rp / dist) - (dist / rp)) / dist2;
-          const real d = mprs[j] - Prs_min;
-          ax += d * w * qx;
-          ay += d * w * qy;
-          if constexpr (DIM == 3) az += d * w * qz;
-        }
-      }
+      for_hits(
+          b, e,
+          [&](int j) {
+            const Vec<real, DIM> pj = mpos[j];
+            return norm2<real, DIM>(pj.x - px, pj.y - py, vz(pj) - pz) < rp2;
+          },
+          [&](int j) {
+            const Vec<real, DIM> pj = mpos[j];
+            const real qx = pj.x - px, qy = pj.y - py, qz = vz(pj) - pz;
+            const real dist2 = norm2<real, DIM>(qx, qy, qz);
+            const real dist = sqrt(dist2);
+            const real w = ((rp / dist) - (dist / rp)) / dist2;
+            const real d = mprs[j] - Prs_min;
+            ax += d * w * qx;
+            ay += d * w * qy;
+            if constexpr (DIM == 3) az += d * w * qz;
+          });
     }
   const real accx = c.invro * ax * c.grad_a;
   const real accy = c.invro * ay * c.grad_a;

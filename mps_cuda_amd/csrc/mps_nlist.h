@@ -1,0 +1,264 @@
+// mps_nlist.h -- ordered neighbour lists shared by ChkCol, MkPrs and
+// PrsGrdTrm.
+//
+// Between the bucket build after UpPcl1 (kernel.cu:1917) and UpPcl2 the
+// fluid positions do not change, and the collision (0.5 l0), pressure and
+// gradient (2.1 l0) radii are all within rp.  One scan of the 27-bucket
+// stencil therefore records, per particle and in the reference's visiting
+// order, every fluid candidate with d2 <= rp2_hi; the three kernels walk that
+// list and apply their own exact radius tests.  A particle with more than
+// kNList candidates is flagged (count -1) and its consumers scan the stencil
+// as before.  Entry k of particle i lives at list[k * n_alloc + i], so the
+// lanes of a wavefront read consecutive words.
+#pragma once
+#include "mps_kernels.h"
+
+namespace mps {
+
+constexpr int kNList = 64;
+
+template <class real, int DIM>
+__global__ __launch_bounds__(kBlock) void k_nlist_build(const Vec<real, DIM>* __restrict__ pos,
+                                                       const int* __restrict__ fcs, int n_alloc,
+                                                       Cst<real> c, int* __restrict__ list,
+                                                       int* __restrict__ count) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_alloc || i >= fcs[c.ncell]) return;
+  const Vec<real, DIM> p = pos[i];
+  const real px = p.x, py = p.y, pz = vz(p), lim = c.rp2_hi;
+  const Cell<real, DIM> q = cell_of<real, DIM>(p, c);
+  const int zlo = DIM == 3 ? q.iz - 1 : 0, zhi = DIM == 3 ? q.iz + 1 : 0;
+  int n = 0;
+  for (int jz = zlo; jz <= zhi; jz++)
+    for (int jy = q.iy - 1; jy <= q.iy + 1; jy++) {
+      const int b = fcs[cell_key(q.ix - 1, jy, jz, c)], e = fcs[cell_key(q.ix + 1, jy, jz, c) + 1];
+      for_hits(
+          b, e,
+          [&](int j) {
+            const Vec<real, DIM> pj = pos[j];
+            return j != i && norm2<real, DIM>(pj.x - px, pj.y - py, vz(pj) - pz) <= lim;
+          },
+          [&](int j) {
+            if (n < kNList) list[(size_t)n * n_alloc + i] = j;
+            n++;
+          });
+    }
+  count[i] = n <= kNList ? n : -1;
+}
+
+// Calls f(j) for the fluid candidates of particle i in visiting order: from
+// the list when it is complete, else from the stencil rows.
+template <class real, int DIM, class F>
+__device__ inline void for_fluid_near(int i, int n_alloc, const int* __restrict__ list,
+                                      const int* __restrict__ count, const int* __restrict__ fcs,
+                                      const Cell<real, DIM>& q, const Cst<real>& c, F&& f) {
+  const int n = count[i];
+  if (n >= 0) {
+    for (int k = 0; k < n; k++) f(list[(size_t)k * n_alloc + i]);
+    return;
+  }
+  const int zlo = DIM == 3 ? q.iz - 1 : 0, zhi = DIM == 3 ? q.iz + 1 : 0;
+  for (int jz = zlo; jz <= zhi; jz++)
+    for (int jy = q.iy - 1; jy <= q.iy + 1; jy++) {
+      const int b = fcs[cell_key(q.ix - 1, jy, jz, c)], e = fcs[cell_key(q.ix + 1, jy, jz, c) + 1];
+      for (int j = b; j < e; j++)
+        if (j != i) f(j);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// d_ChkCol + d_UpChkCol (kernel.cu:1168-1287) over the list.  In each
+// stencil bucket the fluid entries (consecutive in the list) come before
+// the bucket's walls.
+template <class real, int DIM, bool PHASE>
+__global__ __launch_bounds__(kBlock) void k_col_list(
+    const Vec<real, DIM>* __restrict__ pos, const Vec<real, DIM>* __restrict__ vel,
+    const int* __restrict__ fcs, int n_alloc, const int* __restrict__ list,
+    const int* __restrict__ count, const Vec<real, DIM>* __restrict__ wpos,
+    const Vec<real, DIM>* __restrict__ wvel, const int* __restrict__ wcs,
+    const Vec<real, DIM>* __restrict__ mpos, const int* __restrict__ mcs,
+    const uint8_t* __restrict__ wnear, const uint8_t* __restrict__ mnear, Cst<real> c,
+    Vec<real, DIM>* __restrict__ vel_out, Vec<real, DIM>* __restrict__ acc_out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_alloc) return;
+  if (i >= fcs[c.ncell]) {
+    vel_out[i] = vel[i];
+    return;
+  }
+  const Vec<real, DIM> p = pos[i], v = vel[i];
+  const real px = p.x, py = p.y, pz = vz(p);
+  const real vx = v.x, vy = v.y, vvz = vz(v);
+  real v2x = vx, v2y = vy, v2z = vvz;
+  const real mi = c.mass, mj = c.mass;
+  // vel_of(j) is only evaluated for a pair within lim2 (contacts are rare)
+  auto contact = [&](const Vec<real, DIM>& pj, auto&& vel_of, real lim2) {
+    const real qx = pj.x - px, qy = pj.y - py, qz = vz(pj) - pz;
+    const real dist2 = norm2<real, DIM>(qx, qy, qz);
+    if (!(dist2 < lim2)) return;
+    const Vec<real, DIM> u = vel_of();
+    real fDT = dot3<real, DIM>(vx - u.x, vy - u.y, vvz - vz(u), qx, qy, qz);
+    if (fDT > real(0)) {
+      fDT *= c.col * mj / (mi + mj) / dist2;
+      v2x -= qx * fDT;
+      v2y -= qy * fDT;
+      if constexpr (DIM == 3) v2z -= qz * fDT;
+    }
+  };
+  const Cell<real, DIM> q = cell_of<real, DIM>(p, c);
+  const int hk = cell_key(q.ix, q.iy, q.iz, c);
+  const int zlo = DIM == 3 ? q.iz - 1 : 0, zhi = DIM == 3 ? q.iz + 1 : 0;
+  const int n = count[i];
+  int k = 0;
+  if (n >= 0 && !wnear[hk]) {
+    // no wall in the stencil: the list alone is the visiting order
+    for (; k < n; k++) {
+      const int j = list[(size_t)k * n_alloc + i];
+      contact(pos[j], [&] { return vel[j]; }, c.rlim2);
+    }
+  } else
+  for (int jz = zlo; jz <= zhi; jz++)
+    for (int jy = q.iy - 1; jy <= q.iy + 1; jy++)
+      for (int jx = q.ix - 1; jx <= q.ix + 1; jx++) {
+        const int ck = cell_key(jx, jy, jz, c);
+        const int fe = fcs[ck + 1];
+        if (n >= 0) {
+          for (; k < n; k++) {
+            const int j = list[(size_t)k * n_alloc + i];
+            if (j >= fe) break;
+            contact(pos[j], [&] { return vel[j]; }, c.rlim2);
+          }
+        } else {
+          for (int j = fcs[ck]; j < fe; j++)
+            if (j != i) contact(pos[j], [&] { return vel[j]; }, c.rlim2);
+        }
+        for (int j = wcs[ck]; j < wcs[ck + 1]; j++) contact(wpos[j], [&] { return wvel[j]; }, c.rlim_wall2);
+      }
+  if (mnear[hk])
+    for (int jz = zlo; jz <= zhi; jz++)
+      for (int jy = q.iy - 1; jy <= q.iy + 1; jy++) {
+        const int b = mcs[cell_key(q.ix - 1, jy, jz, c)], e = mcs[cell_key(q.ix + 1, jy, jz, c) + 1];
+        for (int j = b; j < e; j++) contact(mpos[j], [&] { return mkvec<real, DIM>(0, 0, 0); }, c.rlim2);
+      }
+  const Vec<real, DIM> out = mkvec<real, DIM>(v2x, v2y, v2z);
+  vel_out[i] = out;
+  if constexpr (PHASE) acc_out[i] = out;
+}
+
+// ---------------------------------------------------------------------------
+// d_MkPrs (kernel.cu:1306-1411) over the list.
+template <class real, int DIM>
+__global__ __launch_bounds__(kBlock) void k_prs_list(
+    const Vec<real, DIM>* __restrict__ pos, const int* __restrict__ fcs, int n_alloc,
+    const int* __restrict__ list, const int* __restrict__ count,
+    const Vec<real, DIM>* __restrict__ mpos, const int* __restrict__ mcs,
+    const uint8_t* __restrict__ mnear, Cst<real> c, real* __restrict__ prs,
+    real* __restrict__ nd) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_alloc || i >= fcs[c.ncell]) return;
+  const Vec<real, DIM> p = pos[i];
+  const real px = p.x, py = p.y, pz = vz(p), rp = c.rp, lim = c.rp2_hi;
+  real ni = 0;
+  auto add = [&](const Vec<real, DIM>& pj) {
+    const real d2 = norm2<real, DIM>(pj.x - px, pj.y - py, vz(pj) - pz);
+    if (!(d2 <= lim)) return;
+    const real dist = sqrt(d2);
+    if (dist < rp) ni += (rp / dist) - (dist / rp);
+  };
+  const Cell<real, DIM> q = cell_of<real, DIM>(p, c);
+  for_fluid_near<real, DIM>(i, n_alloc, list, count, fcs, q, c, [&](int j) { add(pos[j]); });
+  const int zlo = DIM == 3 ? q.iz - 1 : 0, zhi = DIM == 3 ? q.iz + 1 : 0;
+  if (mnear[cell_key(q.ix, q.iy, q.iz, c)])
+  for (int jz = zlo; jz <= zhi; jz++)
+    for (int jy = q.iy - 1; jy <= q.iy + 1; jy++) {
+      const int b = mcs[cell_key(q.ix - 1, jy, jz, c)], e = mcs[cell_key(q.ix + 1, jy, jz, c) + 1];
+      for (int j = b; j < e; j++) add(mpos[j]);
+    }
+  real pressure = real(ni > c.n0_grad) * (ni - c.n0_grad) * c.prs_coef * c.mass;
+  if (pressure <= c.pmax && pressure >= real(0)) {
+  } else {
+    if (pressure > c.pmax) pressure = c.pmax;
+    else pressure = real(0);
+  }
+  prs[i] = pressure;
+  nd[i] = ni;
+}
+
+// ---------------------------------------------------------------------------
+// d_PrsGrdTrm + d_UpPcl2 (kernel.cu:1651-1829) over the list.
+template <class real, int DIM, bool FUSE>
+__global__ __launch_bounds__(kBlock) void k_grad_list(
+    const Vec<real, DIM>* __restrict__ pos, const Vec<real, DIM>* __restrict__ vel,
+    const real* __restrict__ prs, const int* __restrict__ fcs, int n_alloc,
+    const int* __restrict__ list, const int* __restrict__ count,
+    const Vec<real, DIM>* __restrict__ mpos, const real* __restrict__ mprs,
+    const int* __restrict__ mcs, const uint8_t* __restrict__ mnear, Cst<real> c,
+    Vec<real, DIM>* __restrict__ acc_out, Vec<real, DIM>* __restrict__ pos_out,
+    Vec<real, DIM>* __restrict__ vel_out, real* __restrict__ prs_out, real* __restrict__ pav,
+    int8_t* __restrict__ typ) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_alloc) return;
+  if (i >= fcs[c.ncell]) {
+    if constexpr (FUSE) {
+      pos_out[i] = pos[i];
+      vel_out[i] = vel[i];
+      prs_out[i] = prs[i];
+      pav[i] += prs[i];
+    }
+    return;
+  }
+  const Vec<real, DIM> p = pos[i];
+  const real px = p.x, py = p.y, pz = vz(p), rp = c.rp, rp2 = c.rp2;
+  const real prs_i = prs[i];
+  real Prs_min = prs_i;
+  real ax = 0, ay = 0, az = 0;
+  const Cell<real, DIM> q = cell_of<real, DIM>(p, c);
+  for_fluid_near<real, DIM>(i, n_alloc, list, count, fcs, q, c, [&](int j) {
+    const Vec<real, DIM> pj = pos[j];
+    if (norm2<real, DIM>(pj.x - px, pj.y - py, vz(pj) - pz) < rp2) {
+      const real pr = prs[j];
+      if (pr < Prs_min) Prs_min = pr;
+    }
+  });
+  auto add = [&](const Vec<real, DIM>& pj, real pr) {
+    const real qx = pj.x - px, qy = pj.y - py, qz = vz(pj) - pz;
+    const real dist2 = norm2<real, DIM>(qx, qy, qz);
+    if (!(dist2 < rp2)) return;
+    const real dist = sqrt(dist2);
+    const real w = ((rp / dist) - (dist / rp)) / dist2;
+    const real d = pr - Prs_min;
+    ax += d * w * qx;
+    ay += d * w * qy;
+    if constexpr (DIM == 3) az += d * w * qz;
+  };
+  for_fluid_near<real, DIM>(i, n_alloc, list, count, fcs, q, c, [&](int j) { add(pos[j], prs[j]); });
+  const int zlo = DIM == 3 ? q.iz - 1 : 0, zhi = DIM == 3 ? q.iz + 1 : 0;
+  if (mnear[cell_key(q.ix, q.iy, q.iz, c)])
+  for (int jz = zlo; jz <= zhi; jz++)
+    for (int jy = q.iy - 1; jy <= q.iy + 1; jy++) {
+      const int b = mcs[cell_key(q.ix - 1, jy, jz, c)], e = mcs[cell_key(q.ix + 1, jy, jz, c) + 1];
+      for (int j = b; j < e; j++) add(mpos[j], mprs[j]);
+    }
+  const real accx = c.invro * ax * c.grad_a;
+  const real accy = c.invro * ay * c.grad_a;
+  const real accz = DIM == 3 ? c.invro * az * c.grad_a : real(0);
+  if constexpr (!FUSE) {
+    acc_out[i] = mkvec<real, DIM>(accx, accy, accz);
+  } else {
+    const Vec<real, DIM> v = vel[i];
+    real nvx = v.x + accx * c.dt, nvy = v.y + accy * c.dt;
+    real nvz = DIM == 3 ? vz(v) + accz * c.dt : real(0);
+    real nx = px + accx * c.dt * c.dt, ny = py + accy * c.dt * c.dt;
+    real nz = DIM == 3 ? pz + accz * c.dt * c.dt : real(0);
+    pav[i] += prs_i;
+    real po = prs_i;
+    if (!chk_pcl<real, DIM>(nx, ny, nz, nvx, nvy, nvz, c)) {
+      typ[i] = kGST;
+      po = real(0);
+    }
+    pos_out[i] = mkvec<real, DIM>(nx, ny, nz);
+    vel_out[i] = mkvec<real, DIM>(nvx, nvy, nvz);
+    prs_out[i] = po;
+  }
+}
+
+}  // namespace mps
