@@ -171,11 +171,38 @@ int mps_params_default(mps_params* p);
 /* RdDat + AlcBkt + SetPara + Surface_Edge (main.cu:16-22, kernel.cu:1878-1879):
  * validates the scene, computes the constants, uploads the state. */
 int mps_create(const mps_params* params, const mps_scene* scene, mps_handle** out);
+
+/* nscenes independent scenes in one handle, stepped together: every kernel
+ * of mps_step covers all scenes in one launch.  The reference runs one scene
+ * per process (main.cu:10-26); this replaces nscenes such runs.  All scenes
+ * share dim and precision; sizes, ranges and physical parameters may differ.
+ * State arrays of mps_get_state/mps_set_state are the scenes' arrays
+ * concatenated in order (n = sum of scene[k].n); bucket arrays of
+ * mps_get_buckets concatenate the scenes' buckets (ncell = sum of their
+ * ncell, scene k's first bucket at the sum of the ncell before it). */
+int mps_create_batch(int nscenes, const mps_params* params, const mps_scene* scenes,
+                     mps_handle** out);
 int mps_destroy(mps_handle* h);
 
-/* HIP stream (hipStream_t) for all later work; NULL = default stream. */
+/* HIP stream (hipStream_t) for all later work; NULL = the handle's own
+ * stream (created by mps_create). */
 int mps_set_stream(mps_handle* h, void* stream);
 int mps_sync(mps_handle* h);
+
+/* ---- launch configuration (no reference counterpart: the reference's
+ * launch shape is fixed at THREADS per block, const.cuh:8) --------------- */
+
+/* Workgroup size of the per-particle kernels (64, 128 or 256; default 256)
+ * and the workgroup count of the one-wave-per-particle mirror kernel
+ * (default 16384).  Results do not depend on either. */
+int mps_set_launch(mps_handle* h, int block, int wave_grid);
+
+/* steps_per_graph > 0: mps_step records that many steps (rounded up to an
+ * even number) into a hipGraph on first use and replays it for every full
+ * group of steps; the remainder runs eagerly.  0 (default) = eager only.
+ * Results are identical either way.  The graph is re-recorded after
+ * mps_set_state, mps_phase, mps_set_stream or mps_set_launch. */
+int mps_set_graph(mps_handle* h, int steps_per_graph);
 
 /* ---- the hot path ----------------------------------------------------- */
 
@@ -189,6 +216,8 @@ int mps_phase(mps_handle* h, int phase);
 /* ---- state access ----------------------------------------------------- */
 
 int mps_get_constants(mps_handle* h, mps_constants* out);
+/* Constants of scene k of a batch (mps_get_constants = scene 0). */
+int mps_get_scene_constants(mps_handle* h, int scene, mps_constants* out);
 
 /* DevicetoHost (kernel.cu:1847-1869): state in creation order.  Any pointer
  * may be NULL.  Vector fields are [3][n].  acc is the reference's Acc array

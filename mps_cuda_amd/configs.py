@@ -6,6 +6,10 @@
         0.25 x 0.5 x 0.3 (300,000 fluid), jitter +-0.01 l0 (seed 1),
         dt = 2.5e-4, 2000 steps.
   cfg2a 3-D dam break at l0 = 0.0025 (2.4 M fluid), 500 steps.
+  cfg3  512 independent 2-D dam breaks, FP32, l0 = 0.005, tank 1.0 x 0.6;
+        per scene (seed 3): column width U[0.15, 0.35], height U[0.3, 0.6]
+        (1.8k-8.4k fluid), nu log-uniform in [1e-6, 1e-4], c U[15, 30];
+        dt = 1e-4 (acoustic Courant number c dt / l0 <= 0.6), 1000 steps.
 Common constants: rho = 1000, nu = 1e-6, g = -9.81, c = 22 m/s, collision
 distance 0.5 l0, restitution 0.2, re = 3.1 l0, rp = 2.1 l0.
 
@@ -15,6 +19,8 @@ the stated time steps give an acoustic Courant number c*dt/l0 = 1.1 and the
 explicit pressure diverges within 200 steps.
 """
 from __future__ import annotations
+
+import numpy as np
 
 from . import ffi
 
@@ -59,3 +65,48 @@ def small2d(precision="fp64", l0=0.01, jitter=0.0, seed=0, **kw):
     sc = ffi.dambreak(dim=2, l0=l0, tank=(0.3, 0.3, 0.0), column=(0.1, 0.2, 0.0),
                       wall_layers=3, jitter=jitter, seed=seed, domain_top=0.5)
     return sc, params_for(sc, 2, l0, 5e-4, precision, **kw)
+
+
+def dambreak_batch(n_scenes, seed, l0, tank, width, height, nu, c, dt, precision="fp32",
+                   jitter=0.0):
+    """n_scenes 2-D dam breaks with per-scene column, viscosity and sound speed.
+
+    tank, width and height are (lo, hi) ranges in metres (a tank range gives
+    each scene its own tank and bucket grid), nu is log-uniform, c uniform.
+    Returns (scenes, params) lists for ffi.Handle / mps_create_batch.
+    """
+    rng = np.random.default_rng(seed)
+    scenes, params = [], []
+    for k in range(n_scenes):
+        tw = rng.uniform(*tank[0]) if isinstance(tank[0], tuple) else tank[0]
+        th = rng.uniform(*tank[1]) if isinstance(tank[1], tuple) else tank[1]
+        w = rng.uniform(*width)
+        h = rng.uniform(*height)
+        v = float(np.exp(rng.uniform(np.log(nu[0]), np.log(nu[1]))))
+        cc = rng.uniform(*c)
+        sc = ffi.dambreak(dim=2, l0=l0, tank=(tw, th, 0.0), column=(w, h, 0.0), wall_layers=3,
+                          jitter=jitter, seed=seed * 100003 + k, domain_top=th + 0.4)
+        scenes.append(sc)
+        params.append(params_for(sc, 2, l0, dt, precision, nu=v, c=cc))
+    return scenes, params
+
+
+def cfg3(precision="fp32", n_scenes=512):
+    return dambreak_batch(n_scenes, 3, 0.005, (1.0, 0.6), (0.15, 0.35), (0.3, 0.6), (1e-6, 1e-4),
+                          (15.0, 30.0), 1e-4, precision)
+
+
+def batch_small(n_scenes=5, precision="fp64", seed=11, dim=2):
+    """Small uneven scenes (own tank sizes too) for batch parity checks."""
+    if dim == 2:
+        return dambreak_batch(n_scenes, seed, 0.01, ((0.25, 0.45), (0.25, 0.35)), (0.05, 0.15),
+                              (0.1, 0.22), (1e-6, 1e-4), (15.0, 30.0), 2.5e-4, precision,
+                              jitter=0.01)
+    rng = np.random.default_rng(seed)
+    scenes, params = [], []
+    for k in range(n_scenes):
+        sc, p = small3d(precision, seed=seed + k, nu=float(np.exp(rng.uniform(-13.8, -9.2))),
+                        c=rng.uniform(15.0, 30.0))
+        scenes.append(sc)
+        params.append(p)
+    return scenes, params

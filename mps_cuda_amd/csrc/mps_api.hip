@@ -1,9 +1,10 @@
 // mps_api.hip -- C ABI (include/mps.h) over the gfx950 kernels.
 //
-// One handle = one scene on one GPU.  The handle owns every device buffer;
-// all work runs on one HIP stream with no host synchronisation inside
-// mps_step (the reference synchronises after every phase, 18 times a step:
-// kernel.cu:544-1841).
+// A handle holds one scene, or several independent scenes stepped together
+// (mps_create_batch); either way it owns every device buffer and issues all
+// work on one HIP stream, with no host synchronisation inside mps_step (the
+// reference synchronises after every phase, 18 times a step:
+// kernel.cu:544-1841).  A step can be replayed as a captured hipGraph.
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
@@ -45,7 +46,7 @@ inline int nblk(long long n, int cap = 1 << 30) {
 
 struct ImplBase {
   virtual ~ImplBase() {}
-  virtual int init(const mps_params& p, const mps_scene& s) = 0;
+  virtual int init(int S, const mps_params* p, const mps_scene* s) = 0;
   virtual int step(int n) = 0;
   virtual int phase(int ph) = 0;
   virtual int get_state(int8_t*, double*, double*, double*, double*, double*, double*) = 0;
@@ -53,8 +54,12 @@ struct ImplBase {
                         const double*) = 0;
   virtual int buckets(int which, int* cs, int* ids, int* count) = 0;
   virtual int mirrors(int* ids, double* pos, double* prs, int* count) = 0;
-  virtual void constants(mps_constants* out) = 0;
+  virtual int constants(int scene, mps_constants* out) = 0;
+  virtual int set_launch(int block, int wave_grid) = 0;
+  virtual int set_graph(int steps) = 0;
+  virtual int set_stream(hipStream_t s) = 0;
   hipStream_t stream = nullptr;
+  hipStream_t own_stream = nullptr;
   std::vector<void*> allocs;
   int alloc_raw(void** p, size_t bytes) {
     if (bytes == 0) bytes = 256;
@@ -76,6 +81,8 @@ struct ImplBase {
     recs.clear();
     for (hipEvent_t e : pool) (void)hipEventDestroy(e);
     pool.clear();
+    if (own_stream) (void)hipStreamDestroy(own_stream);
+    own_stream = nullptr;
   }
 
   // ---- optional per-kernel timing with HIP events on the handle's stream ----
@@ -158,20 +165,27 @@ struct ImplBase {
 template <class real, int DIM>
 struct Impl : ImplBase {
   using V = mps::Vec<real, DIM>;
-  mps_params prm;
-  mps::HostConst<real> hk;
-  mps::Cst<real> c;
-  int n_total = 0, nf = 0, nw = 0, mcap = 0, ncell = 0;
+  int S = 0;
+  std::vector<mps::HostConst<real>> hks;
+  std::vector<mps::Cst<real>> csts;
+  mps::Cst<real>* dtab = nullptr;  // per-scene constants on the device
+  int* blk_off = nullptr;          // scene-uniform workgroup table [S+1]
+  std::vector<int> nf_scene, nw_scene;  // fluid-set / wall-set entries per scene
+  int block = mps::kBlock;         // workgroup size of the scene-uniform kernels
+  int wave_grid = 16384;           // workgroups of the mirror-wave kernel
+  int sb_grid = 1;                 // workgroups of the scene-uniform kernels
+  int n_total = 0, nf = 0, nw = 0, mcap = 0, tot = 0;
   bool acc_live = false, wall_pav = false;
-  // creation index of each fluid-set / wall-set entry at creation
+  // creation index of each fluid-set / wall-set entry
   std::vector<int> host_fluid_ids, host_wall_ids;
+  std::vector<int8_t> wall_typ;
 
   // fluid set, [0] = current, [1] = scratch
   V *fpos[2], *fvel[2], *facc[2];
   real *fprs[2], *fpav[2], *fnd[2];
   int8_t* ftyp[2];
-  int* fid[2];
-  int* fcs = nullptr;  // ncell + 2
+  int *fid[2], *fsid[2];
+  int* fcs = nullptr;  // tot + 2
   // walls (static)
   V *wpos = nullptr, *wvel = nullptr, *wnrm = nullptr;
   int8_t* wedge = nullptr;
@@ -182,19 +196,28 @@ struct Impl : ImplBase {
   // mirrors
   V *mraw_pos = nullptr, *mpos = nullptr;
   real *mraw_prs = nullptr, *mprs = nullptr;
-  int *mraw_id = nullptr, *mid = nullptr, *mcs = nullptr, *mcount = nullptr;
-  int* near_list = nullptr;  // fluid particles with walls in their stencil
-  int *nlist = nullptr, *ncount = nullptr;  // ordered fluid neighbours within rp
-  uint8_t *wnear = nullptr, *mnear = nullptr;  // any wall / mirror in a bucket's stencil
+  int *mraw_id = nullptr, *mraw_key = nullptr, *mid = nullptr, *mcs = nullptr, *mcount = nullptr;
+  int *near_list = nullptr, *near_scene = nullptr;  // fluid particles with walls in their stencil
+  int *nlist = nullptr, *ncount = nullptr;          // ordered fluid neighbours within rp
+  uint8_t *wnear = nullptr, *mnear = nullptr;       // any wall / mirror in a bucket's stencil
   // bucket scratch (shared by the fluid and mirror builds)
   int *key = nullptr, *slot = nullptr, *tmp = nullptr, *perm = nullptr, *cnt = nullptr;
   // scan levels: parts[l] holds block totals, parts_ex[l] their exclusive scan
   std::vector<int*> parts, parts_ex;
-  std::vector<int> part_n;
+  // captured step graph
+  int graph_steps = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
 
-  ~Impl() override { free_all(); }
+  ~Impl() override {
+    drop_graph();
+    free_all();
+  }
 
-  void fill_cst() {
+  mps::Scenes<real> sc() const { return {dtab, blk_off, S}; }
+
+  static mps::Cst<real> cst_of(const mps::HostConst<real>& hk) {
+    mps::Cst<real> c;
     for (int a = 0; a < 3; a++) {
       c.minc[a] = hk.minc[a];
       c.maxc[a] = hk.maxc[a];
@@ -215,9 +238,9 @@ struct Impl : ImplBase {
     c.rp2_hi = hk.rp2 * (real(1) + real(1.0 / 1024));
     c.col = hk.col;
     c.mass = hk.mass;
-    c.n0knm = (real(1) / hk.n0) * hk.nu;   // kernel.cu:1029, 1113
-    c.grad_a = real(DIM) / hk.n0_grad;     // kernel.cu:1665 (A3 = 3/n0_grad in 3-D)
-    c.invro = real(-1) / hk.mass;          // kernel.cu:1659
+    c.n0knm = (real(1) / hk.n0) * hk.nu;  // kernel.cu:1029, 1113
+    c.grad_a = real(DIM) / hk.n0_grad;    // kernel.cu:1665 (A3 = 3/n0_grad in 3-D)
+    c.invro = real(-1) / hk.mass;         // kernel.cu:1659
     c.dt = hk.dt;
     c.umax = hk.umax;
     c.prs_coef = hk.prs_coef;
@@ -225,107 +248,140 @@ struct Impl : ImplBase {
     c.pmax = hk.pmax;
     c.nbxy = hk.nbxy;
     c.ncell = hk.ncell;
+    c.cbase = 0;
+    c.tot = 0;
+    return c;
   }
 
-  int host_key(const double* pos, int n, int i) const {
+  int host_key(const double* pos, int n, int i, const mps::Cst<real>& c) const {
     V p = mps::mkvec<real, DIM>(real(pos[i]), real(pos[(size_t)n + i]),
                                 DIM == 3 ? real(pos[2 * (size_t)n + i]) : real(0));
     mps::Cell<real, DIM> q = mps::cell_of<real, DIM>(p, c);
     return mps::cell_key(q.ix, q.iy, q.iz, c);
   }
 
-  int init(const mps_params& p, const mps_scene& s) override {
-    prm = p;
+  void update_grid() {
+    long long g = 0;
+    for (int s = 0; s < S; s++) g += (nf_scene[s] + block - 1) / block;
+    sb_grid = (int)std::max<long long>(1, g);
+  }
+
+  int init(int nscene, const mps_params* ps, const mps_scene* ss) override {
+    S = nscene;
     std::string err;
-    int rc = mps::compute_constants<real>(p, s, hk, err);
-    if (rc != MPS_OK) return set_err(rc, err);
-    fill_cst();
-    n_total = s.n;
-    ncell = hk.ncell;
-    std::vector<int8_t> edge;
-    if (s.edge) edge.assign(s.edge, s.edge + s.n);
-    else mps::compute_edge<real>(p, s, edge);
-    for (int i = 0; i < s.n; i++) (mps::is_wall_type(s.typ[i]) ? host_wall_ids : host_fluid_ids).push_back(i);
+    int rc;
+    hks.resize(S);
+    csts.resize(S);
+    std::vector<int> poff(S + 1, 0);
+    for (int s = 0; s < S; s++) {
+      if ((rc = mps::compute_constants<real>(ps[s], ss[s], hks[s], err)))
+        return set_err(rc, S > 1 ? "scene " + std::to_string(s) + ": " + err : err);
+      csts[s] = cst_of(hks[s]);
+      csts[s].cbase = tot;
+      tot += hks[s].ncell;
+      poff[s + 1] = poff[s] + ss[s].n;
+    }
+    for (auto& c : csts) c.tot = tot;
+    n_total = poff[S];
+    // creation index space: scene-major concatenation
+    std::vector<int8_t> typ_all(n_total), edge_all(n_total);
+    std::vector<int> wkey_all(n_total, 0), scene_of(n_total);
+    nf_scene.assign(S, 0);
+    nw_scene.assign(S, 0);
+    for (int s = 0; s < S; s++) {
+      std::vector<int8_t> e;
+      if (ss[s].edge) e.assign(ss[s].edge, ss[s].edge + ss[s].n);
+      else mps::compute_edge<real>(ps[s], ss[s], e);
+      for (int i = 0; i < ss[s].n; i++) {
+        const int g = poff[s] + i;
+        typ_all[g] = ss[s].typ[i];
+        edge_all[g] = e[i];
+        scene_of[g] = s;
+        if (mps::is_wall_type(ss[s].typ[i])) {
+          host_wall_ids.push_back(g);
+          nw_scene[s]++;
+          wkey_all[g] = host_key(ss[s].pos, ss[s].n, i, csts[s]);
+        } else {
+          host_fluid_ids.push_back(g);
+          nf_scene[s]++;
+        }
+      }
+    }
     nf = (int)host_fluid_ids.size();
     nw = (int)host_wall_ids.size();
-    if ((long long)nf * mps::kNumMRR > (1ll << 30))
-      return set_err(MPS_E_ARG, "too many fluid particles for 32-bit mirror ids");
-    // walls sorted by (cell, id), keys computed in `real` exactly as on the device
-    std::vector<int> wkey(s.n, 0);
-    for (int i : host_wall_ids) wkey[i] = host_key(s.pos, s.n, i);
+    if ((long long)n_total * mps::kNumMRR > (1ll << 30))
+      return set_err(MPS_E_ARG, "too many particles for 32-bit mirror ids");
+    // walls sorted by (bucket, id); keys computed in `real` exactly as on the device
     std::stable_sort(host_wall_ids.begin(), host_wall_ids.end(),
-                     [&](int a, int b) { return wkey[a] < wkey[b]; });
+                     [&](int a, int b) { return wkey_all[a] < wkey_all[b]; });
     wall_typ.resize(nw);
-    for (int w = 0; w < nw; w++) wall_typ[w] = s.typ[host_wall_ids[w]];
+    for (int w = 0; w < nw; w++) wall_typ[w] = typ_all[host_wall_ids[w]];
     mcap = std::max(1, nf * mps::kNumMRR);
+    update_grid();
 
+    if (!own_stream) HIP_TRY(hipStreamCreate(&own_stream));
+    stream = own_stream;
     for (int b = 0; b < 2; b++) {
-      if ((rc = alloc(&fpos[b], nf))) return rc;
-      if ((rc = alloc(&fvel[b], nf))) return rc;
-      if ((rc = alloc(&facc[b], nf))) return rc;
-      if ((rc = alloc(&fprs[b], nf))) return rc;
-      if ((rc = alloc(&fpav[b], nf))) return rc;
-      if ((rc = alloc(&fnd[b], nf))) return rc;
-      if ((rc = alloc(&ftyp[b], nf))) return rc;
-      if ((rc = alloc(&fid[b], nf))) return rc;
+      if ((rc = alloc(&fpos[b], nf)) || (rc = alloc(&fvel[b], nf)) || (rc = alloc(&facc[b], nf)) ||
+          (rc = alloc(&fprs[b], nf)) || (rc = alloc(&fpav[b], nf)) || (rc = alloc(&fnd[b], nf)) ||
+          (rc = alloc(&ftyp[b], nf)) || (rc = alloc(&fid[b], nf)) || (rc = alloc(&fsid[b], nf)))
+        return rc;
     }
-    if ((rc = alloc(&fcs, (size_t)ncell + 2))) return rc;
+    if ((rc = alloc(&fcs, (size_t)tot + 2)) || (rc = alloc(&dtab, S)) || (rc = alloc(&blk_off, S + 1)))
+      return rc;
     if ((rc = alloc(&wpos, nw)) || (rc = alloc(&wvel, nw)) || (rc = alloc(&wnrm, nw)) ||
         (rc = alloc(&wedge, nw)) || (rc = alloc(&wcls, nw)) || (rc = alloc(&wprs, nw)) ||
-        (rc = alloc(&wpav, nw)) ||
-        (rc = alloc(&wcs, (size_t)ncell + 1)))
+        (rc = alloc(&wpav, nw)) || (rc = alloc(&wcs, (size_t)tot + 1)))
       return rc;
     if ((rc = alloc(&mraw_pos, mcap)) || (rc = alloc(&mpos, mcap)) || (rc = alloc(&mraw_prs, mcap)) ||
-        (rc = alloc(&mprs, mcap)) || (rc = alloc(&mraw_id, mcap)) || (rc = alloc(&mid, mcap)) ||
-        (rc = alloc(&mcs, (size_t)ncell + 2)) || (rc = alloc(&mcount, 1)) ||
-        (rc = alloc(&near_list, std::max(nf, 1))) || (rc = alloc(&nlist, (size_t)mps::kNList * std::max(nf, 1))) ||
-        (rc = alloc(&ncount, std::max(nf, 1))) || (rc = alloc(&wnear, std::max(ncell, 1))) ||
-        (rc = alloc(&mnear, std::max(ncell, 1))))
+        (rc = alloc(&mprs, mcap)) || (rc = alloc(&mraw_id, mcap)) || (rc = alloc(&mraw_key, mcap)) ||
+        (rc = alloc(&mid, mcap)) || (rc = alloc(&mcs, (size_t)tot + 2)) || (rc = alloc(&mcount, 1)) ||
+        (rc = alloc(&near_list, std::max(nf, 1))) || (rc = alloc(&near_scene, std::max(nf, 1))) ||
+        (rc = alloc(&nlist, (size_t)mps::kNList * std::max(nf, 1))) ||
+        (rc = alloc(&ncount, std::max(nf, 1))) || (rc = alloc(&wnear, std::max(tot, 1))) ||
+        (rc = alloc(&mnear, std::max(tot, 1))))
       return rc;
     const size_t scr = std::max(mcap, nf);
     if ((rc = alloc(&key, scr)) || (rc = alloc(&slot, scr)) || (rc = alloc(&tmp, scr)) ||
-        (rc = alloc(&perm, scr)) || (rc = alloc(&cnt, (size_t)ncell + 1)))
+        (rc = alloc(&perm, scr)) || (rc = alloc(&cnt, (size_t)tot + 1)))
       return rc;
-    // scan levels for n = ncell + 1
-    for (long long n = ncell + 1;;) {
+    for (long long n = tot + 1;;) {  // scan levels for n = tot + 1
       const long long nb = (n + 1 + mps::kScanTile - 1) / mps::kScanTile;
       if (nb <= 1) break;
       int *a, *b;
       if ((rc = alloc(&a, nb)) || (rc = alloc(&b, nb + 1))) return rc;
       parts.push_back(a);
       parts_ex.push_back(b);
-      part_n.push_back((int)nb);
       n = nb;
     }
+    HIP_TRY(hipMemcpy(dtab, csts.data(), S * sizeof(mps::Cst<real>), hipMemcpyHostToDevice));
 
-    // upload walls
-    {
+    auto src = [&](int g, const double* mps_scene::*field, int k) {
+      const int s = scene_of[g], i = g - poff[s];
+      const double* a = ss[s].*field;
+      return a ? real(a[k * (size_t)ss[s].n + i]) : real(0);
+    };
+    auto vec = [&](int g, const double* mps_scene::*field) {
+      return mps::mkvec<real, DIM>(src(g, field, 0), src(g, field, 1),
+                                   DIM == 3 ? src(g, field, 2) : real(0));
+    };
+    {  // walls
       std::vector<V> hp(nw), hv(nw), hn(nw);
       std::vector<int8_t> he(nw);
       std::vector<real> hprs(nw), hpav(nw);
-      std::vector<int> hcs(ncell + 1, 0);
-      const size_t n = s.n;
+      std::vector<int> hcs(tot + 1, 0);
       for (int w = 0; w < nw; w++) {
-        const int i = host_wall_ids[w];
-        auto get = [&](const double* a, int k) { return a ? real(a[k * n + i]) : real(0); };
-        hp[w] = mps::mkvec<real, DIM>(get(s.pos, 0), get(s.pos, 1), DIM == 3 ? get(s.pos, 2) : real(0));
-        hv[w] = mps::mkvec<real, DIM>(get(s.vel, 0), get(s.vel, 1), DIM == 3 ? get(s.vel, 2) : real(0));
-        hn[w] = mps::mkvec<real, DIM>(get(s.normal, 0), get(s.normal, 1),
-                                      DIM == 3 ? get(s.normal, 2) : real(0));
-        he[w] = edge[i];
-        hprs[w] = s.prs ? real(s.prs[i]) : real(0);
-        hpav[w] = s.pav ? real(s.pav[i]) : real(0);
+        const int g = host_wall_ids[w];
+        hp[w] = vec(g, &mps_scene::pos);
+        hv[w] = vec(g, &mps_scene::vel);
+        hn[w] = vec(g, &mps_scene::normal);
+        he[w] = edge_all[g];
+        hprs[w] = src(g, &mps_scene::prs, 0);
+        hpav[w] = src(g, &mps_scene::pav, 0);
         wall_pav = wall_pav || hprs[w] != real(0);
-        hcs[wkey[i] + 1]++;
+        hcs[wkey_all[g] + 1]++;
       }
-      for (int k = 0; k < ncell; k++) hcs[k + 1] += hcs[k];
-      HIP_TRY(hipMemcpy(wpos, hp.data(), nw * sizeof(V), hipMemcpyHostToDevice));
-      HIP_TRY(hipMemcpy(wvel, hv.data(), nw * sizeof(V), hipMemcpyHostToDevice));
-      HIP_TRY(hipMemcpy(wnrm, hn.data(), nw * sizeof(V), hipMemcpyHostToDevice));
-      HIP_TRY(hipMemcpy(wedge, he.data(), nw, hipMemcpyHostToDevice));
-      HIP_TRY(hipMemcpy(wprs, hprs.data(), nw * sizeof(real), hipMemcpyHostToDevice));
-      HIP_TRY(hipMemcpy(wpav, hpav.data(), nw * sizeof(real), hipMemcpyHostToDevice));
-      HIP_TRY(hipMemcpy(wcs, hcs.data(), (ncell + 1) * sizeof(int), hipMemcpyHostToDevice));
+      for (int k = 0; k < tot; k++) hcs[k + 1] += hcs[k];
       // normal classes: walls whose normals compare equal (as the reference's
       // == on each component, kernel.cu:762) share a class
       std::vector<V> cls_n;
@@ -340,46 +396,49 @@ struct Impl : ImplBase {
         if (k >= 64) use_cls = false;
         hcls[w] = uint8_t(k & 63) | (he[w] == MPS_EDGE ? 0x80 : 0);
       }
+      HIP_TRY(hipMemcpy(wpos, hp.data(), nw * sizeof(V), hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(wvel, hv.data(), nw * sizeof(V), hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(wnrm, hn.data(), nw * sizeof(V), hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(wedge, he.data(), nw, hipMemcpyHostToDevice));
       HIP_TRY(hipMemcpy(wcls, hcls.data(), nw, hipMemcpyHostToDevice));
-      hipLaunchKernelGGL((mps::k_cell_near<real, DIM>), dim3(nblk(ncell)), dim3(mps::kBlock), 0, stream,
-                         (const int*)wcs, c, wnear);
+      HIP_TRY(hipMemcpy(wprs, hprs.data(), nw * sizeof(real), hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(wpav, hpav.data(), nw * sizeof(real), hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(wcs, hcs.data(), (tot + 1) * sizeof(int), hipMemcpyHostToDevice));
     }
-    // upload the fluid set in creation order, then bucket it (ClcMPS starts
-    // with MkBkt, kernel.cu:1878)
-    {
-      std::vector<V> hp(nf), hv(nf), ha(nf);
+    {  // the fluid set in creation order; the first bucket build sorts it
+      std::vector<V> hp(nf), hv(nf), ha(nf, mps::mkvec<real, DIM>(0, 0, 0));
       std::vector<real> hprs(nf), hpav(nf), hnd(nf, real(0));
       std::vector<int8_t> ht(nf);
-      const size_t n = s.n;
+      std::vector<int> hsid(nf);
       for (int f = 0; f < nf; f++) {
-        const int i = host_fluid_ids[f];
-        auto get = [&](const double* a, int k) { return a ? real(a[k * n + i]) : real(0); };
-        const bool dead = s.typ[i] == MPS_GST;
-        hp[f] = mps::mkvec<real, DIM>(get(s.pos, 0), get(s.pos, 1), DIM == 3 ? get(s.pos, 2) : real(0));
-        hv[f] = mps::mkvec<real, DIM>(get(s.vel, 0), get(s.vel, 1), DIM == 3 ? get(s.vel, 2) : real(0));
-        ha[f] = mps::mkvec<real, DIM>(0, 0, 0);
-        hprs[f] = s.prs ? real(s.prs[i]) : real(0);
-        hpav[f] = s.pav ? real(s.pav[i]) : real(0);
-        ht[f] = dead ? MPS_GST : MPS_FLD;
+        const int g = host_fluid_ids[f];
+        hp[f] = vec(g, &mps_scene::pos);
+        hv[f] = vec(g, &mps_scene::vel);
+        hprs[f] = src(g, &mps_scene::prs, 0);
+        hpav[f] = src(g, &mps_scene::pav, 0);
+        ht[f] = typ_all[g] == MPS_GST ? MPS_GST : MPS_FLD;
+        hsid[f] = scene_of[g];
       }
+      for (int b = 0; b < 2; b++) HIP_TRY(hipMemcpy(facc[b], ha.data(), nf * sizeof(V), hipMemcpyHostToDevice));
       HIP_TRY(hipMemcpy(fpos[0], hp.data(), nf * sizeof(V), hipMemcpyHostToDevice));
       HIP_TRY(hipMemcpy(fvel[0], hv.data(), nf * sizeof(V), hipMemcpyHostToDevice));
-      HIP_TRY(hipMemcpy(facc[0], ha.data(), nf * sizeof(V), hipMemcpyHostToDevice));
-      HIP_TRY(hipMemcpy(facc[1], ha.data(), nf * sizeof(V), hipMemcpyHostToDevice));
       HIP_TRY(hipMemcpy(fprs[0], hprs.data(), nf * sizeof(real), hipMemcpyHostToDevice));
       HIP_TRY(hipMemcpy(fpav[0], hpav.data(), nf * sizeof(real), hipMemcpyHostToDevice));
       HIP_TRY(hipMemcpy(fnd[0], hnd.data(), nf * sizeof(real), hipMemcpyHostToDevice));
       HIP_TRY(hipMemcpy(ftyp[0], ht.data(), nf, hipMemcpyHostToDevice));
       HIP_TRY(hipMemcpy(fid[0], host_fluid_ids.data(), nf * sizeof(int), hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(fsid[0], hsid.data(), nf * sizeof(int), hipMemcpyHostToDevice));
     }
-    hipLaunchKernelGGL(mps::k_fill_int, dim3(nblk(ncell + 2, 4096)), dim3(mps::kBlock), 0, stream,
-                       mcs, ncell + 2, 0);
-    hipLaunchKernelGGL(mps::k_fill_int, dim3(1), dim3(mps::kBlock), 0, stream, mcount, 1, 0);
-    hipLaunchKernelGGL((mps::k_fill<uint8_t>), dim3(nblk(ncell, 4096)), dim3(mps::kBlock), 0, stream,
-                       mnear, ncell, (uint8_t)0);
-    if ((rc = bucket())) return rc;
+    const dim3 B(mps::kBlock);
+    hipLaunchKernelGGL((mps::k_cell_near<real, DIM>), dim3(nblk(tot)), B, 0, stream, (const int*)wcs,
+                       (const mps::Cst<real>*)dtab, S, tot, wnear);
+    hipLaunchKernelGGL(mps::k_fill_int, dim3(nblk(tot + 2, 4096)), B, 0, stream, mcs, tot + 2, 0);
+    hipLaunchKernelGGL(mps::k_fill_int, dim3(1), B, 0, stream, mcount, 1, 0);
+    hipLaunchKernelGGL((mps::k_fill<uint8_t>), dim3(nblk(tot, 4096)), B, 0, stream, mnear, tot,
+                       (uint8_t)0);
+    if ((rc = bucket())) return rc;  // ClcMPS starts with MkBkt (kernel.cu:1878)
     HIP_TRY(hipStreamSynchronize(stream));
-    return MPS_OK;
+    return check_launch();
   }
 
   // exclusive scan of in[0..n) into out[0..n] (out[n] = total)
@@ -397,7 +456,8 @@ struct Impl : ImplBase {
   }
 
   mps::FluidArrays<real, DIM> fa(int b) {
-    return {fpos[b], fvel[b], acc_live ? facc[b] : nullptr, fprs[b], fpav[b], fnd[b], ftyp[b], fid[b]};
+    return {fpos[b], fvel[b], acc_live ? facc[b] : nullptr, fprs[b], fpav[b],
+            fnd[b],  ftyp[b], fid[b],                       fsid[b]};
   }
   void swap_all() {
     std::swap(fpos[0], fpos[1]);
@@ -408,66 +468,72 @@ struct Impl : ImplBase {
     std::swap(fnd[0], fnd[1]);
     std::swap(ftyp[0], ftyp[1]);
     std::swap(fid[0], fid[1]);
+    std::swap(fsid[0], fsid[1]);
   }
 
-  // MkBkt (kernel.cu:532-550) as count / scan / scatter / rank / gather.
+  // MkBkt (kernel.cu:532-550) as count / scan / scatter / rank / gather, then
+  // the scene-uniform workgroup table of the new order.
   int bucket() {
+    const dim3 B(mps::kBlock);
     if (nf == 0) {
-      hipLaunchKernelGGL(mps::k_fill_int, dim3(nblk(ncell + 2, 4096)), dim3(mps::kBlock), 0, stream,
-                         fcs, ncell + 2, 0);
+      hipLaunchKernelGGL(mps::k_fill_int, dim3(nblk(tot + 2, 4096)), B, 0, stream, fcs, tot + 2, 0);
+      hipLaunchKernelGGL(mps::k_fill_int, dim3(1), B, 0, stream, blk_off, S + 1, 0);
       return MPS_OK;
     }
-    const dim3 B(mps::kBlock);
-    hipLaunchKernelGGL(mps::k_fill_int, dim3(nblk(ncell + 1, 4096)), B, 0, stream, cnt, ncell + 1, 0);
+    hipLaunchKernelGGL(mps::k_fill_int, dim3(nblk(tot + 1, 4096)), B, 0, stream, cnt, tot + 1, 0);
     TLAUNCH("bucket_key", (mps::k_bkt_key<real, DIM>), dim3(nblk(nf)), B, 0, stream, fpos[0],
-                       (const int8_t*)ftyp[0], nf, (const int*)nullptr, c, key, slot, cnt);
+            (const int8_t*)ftyp[0], (const int*)fsid[0], (const mps::Cst<real>*)dtab, nf, tot, key,
+            slot, cnt);
     tick("bucket_scan");
-    scan(cnt, ncell + 1, fcs, 0);
+    scan(cnt, tot + 1, fcs, 0);
     tock();
     TLAUNCH("bucket_scatter", mps::k_bkt_scatter, dim3(nblk(nf)), B, 0, stream, key, slot, fcs, nf,
-                       (const int*)nullptr, 1, tmp);
+            (const int*)nullptr, 1, tmp);
     TLAUNCH("bucket_rank", mps::k_bkt_rank, dim3(nblk(nf)), B, 0, stream, tmp, key, fid[0], fcs, nf,
-                       (const int*)nullptr, 1, ncell, perm);
+            (const int*)nullptr, tot, perm);
     TLAUNCH("bucket_gather", (mps::k_gather_fluid<real, DIM>), dim3(nblk(nf)), B, 0, stream, perm, nf,
-                       fa(0), fa(1));
+            fa(0), fa(1));
     swap_all();
+    hipLaunchKernelGGL((mps::k_scene_blocks<real>), dim3(1), B, 0, stream, (const mps::Cst<real>*)dtab,
+                       S, (const int*)fcs, block, blk_off);
     return MPS_OK;
   }
 
   // ResetMRR + GenMRR_nonslip + MkBkt_MRR (kernel.cu:1906-1908).
   int mirror() {
-    const dim3 B(mps::kBlock);
+    const dim3 B(mps::kBlock), SB(block), SG(sb_grid);
     hipLaunchKernelGGL(mps::k_fill_int, dim3(1), B, 0, stream, mcount, 1, 0);
     // raw slots: 10 per listed particle (wave path) or compacted (general path)
     int mult = 1;
     if (nf > 0 && use_cls) {
-      TLAUNCH("mirror_near", (mps::k_mirror_near<real, DIM>), dim3(nblk(nf)), B, 0, stream, fpos[0],
-              fcs, nf, (const uint8_t*)wnear, c, near_list, mcount);
-      TLAUNCH("mirror_gen", (mps::k_mirror_wave<real, DIM>), dim3(std::min(nf, 16384)), dim3(64), 0,
-              stream, fpos[0], fprs[0], fid[0], near_list, mcount, wpos, wnrm, wcls, wcs, c,
-              mraw_pos, mraw_prs, mraw_id);
+      TLAUNCH("mirror_near", (mps::k_mirror_near<real, DIM>), SG, SB, 0, stream, fpos[0],
+              (const int*)fcs, sc(), (const uint8_t*)wnear, near_list, near_scene, mcount);
+      TLAUNCH("mirror_gen", (mps::k_mirror_wave<real, DIM>), dim3(std::max(1, std::min(nf, wave_grid))),
+              dim3(64), 0, stream, fpos[0], fprs[0], fid[0], near_list, near_scene, mcount,
+              (const mps::Cst<real>*)dtab, wpos, wnrm, wcls, wcs, mraw_pos, mraw_prs, mraw_id,
+              mraw_key);
       mult = mps::kNumMRR;
     } else if (nf > 0) {
-      TLAUNCH("mirror_gen", (mps::k_mirror_gen<real, DIM>), dim3(nblk(nf)), B, 0, stream, fpos[0],
-              fprs[0], fid[0], fcs, nf, wpos, wnrm, wedge, wcs, c, mraw_pos, mraw_prs, mraw_id,
+      TLAUNCH("mirror_gen", (mps::k_mirror_gen<real, DIM>), SG, SB, 0, stream, fpos[0], fprs[0], fid[0],
+              (const int*)fcs, sc(), wpos, wnrm, wedge, wcs, mraw_pos, mraw_prs, mraw_id, mraw_key,
               mcount, mcap);
     }
     const int g = nblk(mcap, 2048);
-    hipLaunchKernelGGL(mps::k_fill_int, dim3(nblk(ncell + 1, 4096)), B, 0, stream, cnt, ncell + 1, 0);
-    TLAUNCH("mirror_key", (mps::k_mirror_key<real, DIM>), dim3(g), B, 0, stream, mraw_pos,
-            (const int*)mraw_id, (const int*)mcount, mult, c, key, slot, cnt);
+    hipLaunchKernelGGL(mps::k_fill_int, dim3(nblk(tot + 1, 4096)), B, 0, stream, cnt, tot + 1, 0);
+    TLAUNCH("mirror_key", mps::k_key_count, dim3(g), B, 0, stream, (const int*)mraw_key,
+            (const int*)mcount, mult, slot, cnt);
     tick("mirror_scan");
-    scan(cnt, ncell + 1, mcs, 0);
+    scan(cnt, tot + 1, mcs, 0);
     tock();
-    TLAUNCH("mirror_scatter", mps::k_bkt_scatter, dim3(g), B, 0, stream, key, slot, mcs, 0,
-            (const int*)mcount, mult, tmp);
-    TLAUNCH("mirror_rank", mps::k_bkt_rank, dim3(g), B, 0, stream, tmp, key, mraw_id, mcs, 0,
-            (const int*)(mcs + ncell), 1, ncell, perm);
-    // live mirrors: mcs[ncell] (empty slots sit in the tail cell)
+    TLAUNCH("mirror_scatter", mps::k_bkt_scatter, dim3(g), B, 0, stream, (const int*)mraw_key, slot, mcs,
+            0, (const int*)mcount, mult, tmp);
+    // live mirrors: mcs[tot]
+    TLAUNCH("mirror_rank", mps::k_bkt_rank, dim3(g), B, 0, stream, tmp, (const int*)mraw_key, mraw_id, mcs,
+            0, (const int*)(mcs + tot), tot, perm);
     TLAUNCH("mirror_gather", (mps::k_gather_mirror<real, DIM>), dim3(g), B, 0, stream, perm,
-            (const int*)(mcs + ncell), mraw_pos, mraw_prs, mraw_id, mpos, mprs, mid);
-    TLAUNCH("mirror_flags", (mps::k_cell_near<real, DIM>), dim3(nblk(ncell)), B, 0, stream,
-            (const int*)mcs, c, mnear);
+            (const int*)(mcs + tot), mraw_pos, mraw_prs, mraw_id, mpos, mprs, mid);
+    TLAUNCH("mirror_flags", (mps::k_cell_near<real, DIM>), dim3(nblk(tot)), B, 0, stream,
+            (const int*)mcs, (const mps::Cst<real>*)dtab, S, tot, mnear);
     return MPS_OK;
   }
 
@@ -478,69 +544,78 @@ struct Impl : ImplBase {
   }
 
   // ---- phase launchers ------------------------------------------------------
+  void copy_tail(V* a_in, V* a_out, V* b_in, V* b_out, real* s_in, real* s_out, real* pav) {
+    hipLaunchKernelGGL((mps::k_copy_tail<real, DIM>), dim3(nblk(nf, 1024)), dim3(mps::kBlock), 0, stream,
+                       (const int*)fcs, tot, nf, (const V*)a_in, a_out, (const V*)b_in, b_out,
+                       (const real*)s_in, s_out, pav);
+  }
+
   // d_VscTrm (+ d_UpPcl1 when fuse): fuse writes the new pos/vel and swaps
   // them in; otherwise the acceleration goes to facc.
   void run_vsc(bool fuse) {
-    const dim3 B(mps::kBlock), G(nblk(nf));
+    const dim3 SB(block), SG(sb_grid);
     if (fuse) {
-      TLAUNCH("vsc_up1", (mps::k_vsc<real, DIM, true>), G, B, 0, stream, fpos[0], fvel[0], fcs, nf,
-              mpos, mcs, (const uint8_t*)mnear, c, (V*)nullptr, fpos[1], fvel[1], ftyp[0], fprs[0]);
+      TLAUNCH("vsc_up1", (mps::k_vsc<real, DIM, true>), SG, SB, 0, stream, fpos[0], fvel[0],
+              (const int*)fcs, sc(), mpos, mcs, (const uint8_t*)mnear, (V*)nullptr, fpos[1], fvel[1],
+              ftyp[0], fprs[0]);
+      copy_tail(fpos[0], fpos[1], fvel[0], fvel[1], nullptr, nullptr, nullptr);
       std::swap(fpos[0], fpos[1]);
       std::swap(fvel[0], fvel[1]);
     } else {
-      TLAUNCH("vsc", (mps::k_vsc<real, DIM, false>), G, B, 0, stream, fpos[0], fvel[0], fcs, nf, mpos,
-              mcs, (const uint8_t*)mnear, c, facc[0], (V*)nullptr, (V*)nullptr, (int8_t*)nullptr, (real*)nullptr);
+      TLAUNCH("vsc", (mps::k_vsc<real, DIM, false>), SG, SB, 0, stream, fpos[0], fvel[0],
+              (const int*)fcs, sc(), mpos, mcs, (const uint8_t*)mnear, facc[0], (V*)nullptr, (V*)nullptr,
+              (int8_t*)nullptr, (real*)nullptr);
     }
   }
 
   // Ordered fluid neighbour lists within rp for ChkCol/MkPrs/PrsGrdTrm.
   void run_nlist() {
-    TLAUNCH("nlist", (mps::k_nlist_build<real, DIM>), dim3(nblk(nf)), dim3(mps::kBlock), 0, stream,
-            fpos[0], fcs, nf, c, nlist, ncount);
+    TLAUNCH("nlist", (mps::k_nlist_build<real, DIM>), dim3(sb_grid), dim3(block), 0, stream, fpos[0],
+            (const int*)fcs, nf, sc(), nlist, ncount);
   }
 
   // d_ChkCol + d_UpChkCol: new velocities into fvel[1], swapped in.
   void run_col(bool phase_mode) {
-    const dim3 B(mps::kBlock), G(nblk(nf));
+    const dim3 SB(block), SG(sb_grid);
     if (phase_mode)
-      TLAUNCH("col", (mps::k_col_list<real, DIM, true>), G, B, 0, stream, fpos[0], fvel[0], fcs, nf,
-              nlist, ncount, wpos, wvel, wcs, mpos, mcs, (const uint8_t*)wnear,
-              (const uint8_t*)mnear, c, fvel[1], facc[0]);
+      TLAUNCH("col", (mps::k_col_list<real, DIM, true>), SG, SB, 0, stream, fpos[0], fvel[0],
+              (const int*)fcs, nf, nlist, ncount, wpos, wvel, wcs, mpos, mcs, (const uint8_t*)wnear,
+              (const uint8_t*)mnear, sc(), fvel[1], facc[0]);
     else
-      TLAUNCH("col", (mps::k_col_list<real, DIM, false>), G, B, 0, stream, fpos[0], fvel[0], fcs, nf,
-              nlist, ncount, wpos, wvel, wcs, mpos, mcs, (const uint8_t*)wnear,
-              (const uint8_t*)mnear, c, fvel[1], (V*)nullptr);
+      TLAUNCH("col", (mps::k_col_list<real, DIM, false>), SG, SB, 0, stream, fpos[0], fvel[0],
+              (const int*)fcs, nf, nlist, ncount, wpos, wvel, wcs, mpos, mcs, (const uint8_t*)wnear,
+              (const uint8_t*)mnear, sc(), fvel[1], (V*)nullptr);
+    copy_tail(fvel[0], fvel[1], nullptr, nullptr, nullptr, nullptr, nullptr);
     std::swap(fvel[0], fvel[1]);
   }
 
   void run_prs() {
-    TLAUNCH("prs", (mps::k_prs_list<real, DIM>), dim3(nblk(nf)), dim3(mps::kBlock), 0, stream, fpos[0],
-            fcs, nf, nlist, ncount, mpos, mcs, (const uint8_t*)mnear, c, fprs[0], fnd[0]);
+    TLAUNCH("prs", (mps::k_prs_list<real, DIM>), dim3(sb_grid), dim3(block), 0, stream, fpos[0],
+            (const int*)fcs, nf, nlist, ncount, mpos, mcs, (const uint8_t*)mnear, sc(), fprs[0], fnd[0]);
   }
 
   // d_PrsGrdTrm (+ d_UpPcl2 when fuse: new pos/vel/prs swapped in, pav += prs).
   void run_grad(bool fuse) {
-    const dim3 B(mps::kBlock), G(nblk(nf));
+    const dim3 SB(block), SG(sb_grid);
     if (fuse) {
-      TLAUNCH("grad_up2", (mps::k_grad_list<real, DIM, true>), G, B, 0, stream, fpos[0], fvel[0],
-              fprs[0], fcs, nf, nlist, ncount, mpos, mprs, mcs, (const uint8_t*)mnear, c, (V*)nullptr,
-              fpos[1], fvel[1],
-              fprs[1], fpav[0], ftyp[0]);
+      TLAUNCH("grad_up2", (mps::k_grad_list<real, DIM, true>), SG, SB, 0, stream, fpos[0], fvel[0],
+              fprs[0], (const int*)fcs, nf, nlist, ncount, mpos, mprs, mcs, (const uint8_t*)mnear, sc(),
+              (V*)nullptr, fpos[1], fvel[1], fprs[1], fpav[0], ftyp[0]);
+      copy_tail(fpos[0], fpos[1], fvel[0], fvel[1], fprs[0], fprs[1], fpav[0]);
       std::swap(fpos[0], fpos[1]);
       std::swap(fvel[0], fvel[1]);
       std::swap(fprs[0], fprs[1]);
     } else {
-      TLAUNCH("grad", (mps::k_grad_list<real, DIM, false>), G, B, 0, stream, fpos[0], fvel[0], fprs[0],
-              fcs, nf, nlist, ncount, mpos, mprs, mcs, (const uint8_t*)mnear, c, facc[0],
-              (V*)nullptr, (V*)nullptr,
-              (real*)nullptr, (real*)nullptr, (int8_t*)nullptr);
+      TLAUNCH("grad", (mps::k_grad_list<real, DIM, false>), SG, SB, 0, stream, fpos[0], fvel[0], fprs[0],
+              (const int*)fcs, nf, nlist, ncount, mpos, mprs, mcs, (const uint8_t*)mnear, sc(), facc[0],
+              (V*)nullptr, (V*)nullptr, (real*)nullptr, (real*)nullptr, (int8_t*)nullptr);
     }
   }
 
   // One ClcMPS loop body (kernel.cu:1904-1935) with fused kernels.  The
   // redundant MkBkt of kernel.cu:1921 is skipped: positions have not
   // changed since kernel.cu:1917, so it would rebuild identical buckets.
-  int step(int nsteps) override {
+  void step_eager(int nsteps) {
     const dim3 B(mps::kBlock);
     if (acc_live) {
       hipLaunchKernelGGL((mps::k_fill<V>), dim3(nblk(nf, 4096)), B, 0, stream, facc[0], nf,
@@ -562,11 +637,77 @@ struct Impl : ImplBase {
       if (wall_pav && nw > 0)
         hipLaunchKernelGGL((mps::k_wall_pav<real>), dim3(nblk(nw)), B, 0, stream, wpav, wprs, nw);
     }
+  }
+
+  void drop_graph() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    gexec = nullptr;
+    graph = nullptr;
+  }
+
+  // Captures graph_steps steps.  An even count returns every double-buffered
+  // field to the buffer it started in, so the graph can be replayed.
+  std::vector<const void*> buffers() const {
+    return {fpos[0], fvel[0], facc[0], fprs[0], fpav[0], fnd[0], ftyp[0], fid[0], fsid[0]};
+  }
+  std::vector<const void*> graph_buffers;  // buffer order the graph was captured with
+
+  int capture() {
+    graph_buffers = buffers();
+    HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    step_eager(graph_steps);
+    HIP_TRY(hipStreamEndCapture(stream, &graph));
+    if (buffers() != graph_buffers) {
+      drop_graph();
+      return set_err(MPS_E_STATE, "captured steps do not return to the starting buffers");
+    }
+    HIP_TRY(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+    return MPS_OK;
+  }
+
+  int step(int nsteps) override {
+    if (graph_steps > 0 && !timing && nsteps >= graph_steps) {
+      if (acc_live) step_eager(0);  // clears the phase-mode acceleration
+      // an odd number of eager steps since the capture swapped the buffers
+      if (gexec && buffers() != graph_buffers) drop_graph();
+      int rc;
+      if (!gexec && (rc = capture())) return rc;
+      for (; nsteps >= graph_steps; nsteps -= graph_steps) HIP_TRY(hipGraphLaunch(gexec, stream));
+    }
+    step_eager(nsteps);
     return check_launch();
+  }
+
+  int set_graph(int steps) override {
+    if (steps < 0) return set_err(MPS_E_ARG, "negative step count");
+    drop_graph();
+    graph_steps = steps + (steps & 1);
+    return MPS_OK;
+  }
+
+  int set_launch(int b, int wg) override {
+    if (b != 64 && b != 128 && b != 256) return set_err(MPS_E_ARG, "block must be 64, 128 or 256");
+    if (wg < 1) return set_err(MPS_E_ARG, "wave grid must be positive");
+    drop_graph();
+    block = b;
+    wave_grid = wg;
+    update_grid();
+    hipLaunchKernelGGL((mps::k_scene_blocks<real>), dim3(1), dim3(mps::kBlock), 0, stream,
+                       (const mps::Cst<real>*)dtab, S, (const int*)fcs, block, blk_off);
+    return check_launch();
+  }
+
+  int set_stream(hipStream_t s) override {
+    HIP_TRY(hipStreamSynchronize(stream));
+    drop_graph();
+    stream = s ? s : own_stream;
+    return MPS_OK;
   }
 
   int phase(int ph) override {
     const dim3 B(mps::kBlock), G(nblk(nf));
+    drop_graph();  // host-side buffer order may change
     acc_live = true;
     switch (ph) {
       case MPS_PH_MKBKT: bucket(); break;
@@ -576,8 +717,8 @@ struct Impl : ImplBase {
         break;
       case MPS_PH_UP1:
         if (nf > 0)
-          hipLaunchKernelGGL((mps::k_up1<real, DIM>), G, B, 0, stream, fpos[0], fvel[0], facc[0],
-                             ftyp[0], fprs[0], nf, c);
+          hipLaunchKernelGGL((mps::k_up1<real, DIM>), G, B, 0, stream, fpos[0], fvel[0], facc[0], ftyp[0],
+                             fprs[0], (const int*)fsid[0], (const mps::Cst<real>*)dtab, nf);
         break;
       case MPS_PH_COL:
         if (nf > 0) {
@@ -599,8 +740,8 @@ struct Impl : ImplBase {
         break;
       case MPS_PH_UP2:
         if (nf > 0)
-          hipLaunchKernelGGL((mps::k_up2<real, DIM>), G, B, 0, stream, fpos[0], fvel[0], facc[0],
-                             ftyp[0], fprs[0], fpav[0], nf, c);
+          hipLaunchKernelGGL((mps::k_up2<real, DIM>), G, B, 0, stream, fpos[0], fvel[0], facc[0], ftyp[0],
+                             fprs[0], fpav[0], (const int*)fsid[0], (const mps::Cst<real>*)dtab, nf);
         if (wall_pav && nw > 0)
           hipLaunchKernelGGL((mps::k_wall_pav<real>), dim3(nblk(nw)), B, 0, stream, wpav, wprs, nw);
         break;
@@ -660,12 +801,12 @@ struct Impl : ImplBase {
     }
     return MPS_OK;
   }
-  std::vector<int8_t> wall_typ;
 
   int set_state(const int8_t* typ, const double* pos, const double* vel, const double* prs,
                 const double* pav) override {
     std::vector<int> hid;
     int rc;
+    drop_graph();
     if ((rc = d2h(hid, fid[0], nf))) return rc;
     HIP_TRY(hipStreamSynchronize(stream));
     const size_t n = n_total;
@@ -720,9 +861,9 @@ struct Impl : ImplBase {
     const int* dcs = which == MPS_BKT_FLUID ? fcs : which == MPS_BKT_WALL ? wcs : mcs;
     std::vector<int> hcs;
     int rc;
-    if ((rc = d2h(hcs, dcs, (size_t)ncell + 1))) return rc;
+    if ((rc = d2h(hcs, dcs, (size_t)tot + 1))) return rc;
     HIP_TRY(hipStreamSynchronize(stream));
-    const int n = hcs[ncell];
+    const int n = hcs[tot];
     *count = n;
     if (cs) std::copy(hcs.begin(), hcs.end(), cs);
     if (ids) {
@@ -761,16 +902,39 @@ struct Impl : ImplBase {
     return MPS_OK;
   }
 
-  void constants(mps_constants* out) override {
-    mps::fill_constants_out(hk, DIM, out);
-    out->n_fluid = nf;
-    out->n_wall = nw;
+  int constants(int scene, mps_constants* out) override {
+    if (scene < 0 || scene >= S) return set_err(MPS_E_ARG, "scene index out of range");
+    mps::fill_constants_out(hks[scene], DIM, out);
+    out->n_fluid = nf_scene[scene];
+    out->n_wall = nw_scene[scene];
+    return MPS_OK;
   }
 };
 
 template <class real, int DIM>
 ImplBase* make_impl() {
   return new Impl<real, DIM>();
+}
+
+int create_handle(int nscene, const mps_params* p, const mps_scene* s, ImplBase** out) {
+  std::string err;
+  for (int k = 0; k < nscene; k++) {
+    int rc = mps::validate(p + k, s + k, err);
+    if (rc != MPS_OK) return set_err(rc, nscene > 1 ? "scene " + std::to_string(k) + ": " + err : err);
+    if (p[k].dim != p[0].dim || p[k].precision != p[0].precision)
+      return set_err(MPS_E_ARG, "all scenes of a batch need the same dim and precision");
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return set_err(MPS_E_HIP, "no HIP device available");
+  ImplBase* b;
+  if (p->precision == MPS_FP64) b = p->dim == 3 ? make_impl<double, 3>() : make_impl<double, 2>();
+  else b = p->dim == 3 ? make_impl<float, 3>() : make_impl<float, 2>();
+  std::unique_ptr<ImplBase> hold(b);
+  int rc = b->init(nscene, p, s);
+  if (rc != MPS_OK) return rc;
+  *out = hold.release();
+  return MPS_OK;
 }
 
 }  // namespace
@@ -786,19 +950,22 @@ const char* mps_last_error(void) { return g_err.c_str(); }
 int mps_create(const mps_params* p, const mps_scene* s, mps_handle** out) {
   if (!out) return set_err(MPS_E_ARG, "null output handle");
   *out = nullptr;
-  std::string err;
-  int rc = mps::validate(p, s, err);
-  if (rc != MPS_OK) return set_err(rc, err);
-  int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
-    return set_err(MPS_E_HIP, "no HIP device available");
-  ImplBase* b;
-  if (p->precision == MPS_FP64) b = p->dim == 3 ? make_impl<double, 3>() : make_impl<double, 2>();
-  else b = p->dim == 3 ? make_impl<float, 3>() : make_impl<float, 2>();
-  std::unique_ptr<mps_handle> h(new mps_handle{std::unique_ptr<ImplBase>(b)});
-  rc = b->init(*p, *s);
+  if (!p || !s) return set_err(MPS_E_ARG, "null params or scene");
+  ImplBase* b = nullptr;
+  int rc = create_handle(1, p, s, &b);
   if (rc != MPS_OK) return rc;
-  *out = h.release();
+  *out = new mps_handle{std::unique_ptr<ImplBase>(b)};
+  return MPS_OK;
+}
+
+int mps_create_batch(int nscenes, const mps_params* p, const mps_scene* s, mps_handle** out) {
+  if (!out) return set_err(MPS_E_ARG, "null output handle");
+  *out = nullptr;
+  if (nscenes < 1 || !p || !s) return set_err(MPS_E_ARG, "need at least one scene");
+  ImplBase* b = nullptr;
+  int rc = create_handle(nscenes, p, s, &b);
+  if (rc != MPS_OK) return rc;
+  *out = new mps_handle{std::unique_ptr<ImplBase>(b)};
   return MPS_OK;
 }
 
@@ -809,8 +976,7 @@ int mps_destroy(mps_handle* h) {
 
 int mps_set_stream(mps_handle* h, void* stream) {
   if (!h) return set_err(MPS_E_ARG, "null handle");
-  h->impl->stream = (hipStream_t)stream;
-  return MPS_OK;
+  return h->impl->set_stream((hipStream_t)stream);
 }
 
 int mps_sync(mps_handle* h) {
@@ -829,10 +995,24 @@ int mps_phase(mps_handle* h, int phase) {
   return h->impl->phase(phase);
 }
 
+int mps_set_graph(mps_handle* h, int steps_per_graph) {
+  if (!h) return set_err(MPS_E_ARG, "null handle");
+  return h->impl->set_graph(steps_per_graph);
+}
+
+int mps_set_launch(mps_handle* h, int block, int wave_grid) {
+  if (!h) return set_err(MPS_E_ARG, "null handle");
+  return h->impl->set_launch(block, wave_grid);
+}
+
 int mps_get_constants(mps_handle* h, mps_constants* out) {
   if (!h || !out) return set_err(MPS_E_ARG, "null argument");
-  h->impl->constants(out);
-  return MPS_OK;
+  return h->impl->constants(0, out);
+}
+
+int mps_get_scene_constants(mps_handle* h, int scene, mps_constants* out) {
+  if (!h || !out) return set_err(MPS_E_ARG, "null argument");
+  return h->impl->constants(scene, out);
 }
 
 int mps_get_state(mps_handle* h, int8_t* typ, double* pos, double* vel, double* acc, double* prs,

@@ -60,6 +60,20 @@ struct Cst {
   real r2_hi, rp2_hi;
   real col, mass, n0knm, grad_a, invro, dt, umax, prs_coef, n0_grad, pmax;
   int nb[3], nbxy, ncell;
+  // A handle holds one or more independent scenes.  Their buckets share one
+  // index space: scene s owns keys [cbase, cbase + ncell); `tot` (the number
+  // of buckets of all scenes) is the key of deleted particles.
+  int cbase, tot;
+};
+
+// Scene lookup for kernels launched with scene-uniform workgroups: block b
+// serves scene s when blk_off[s] <= b < blk_off[s+1], and covers live
+// fluid entries cs[cbase_s] + (b - blk_off[s]) * blockDim.x + threadIdx.x.
+template <class real>
+struct Scenes {
+  const Cst<real>* tab;   // [S]
+  const int* blk_off;     // [S+1]
+  int S;
 };
 
 // Bucket cell coordinate, kernel.cu:521-523, clamped to [1, nB-2].
@@ -89,7 +103,28 @@ __host__ __device__ inline Cell<real, DIM> cell_of(const Vec<real, DIM>& p, cons
 
 template <class real>
 __host__ __device__ inline int cell_key(int ix, int iy, int iz, const Cst<real>& c) {
-  return iz * c.nbxy + iy * c.nb[0] + ix;
+  return c.cbase + iz * c.nbxy + iy * c.nb[0] + ix;
+}
+
+// Scene of workgroup blockIdx.x (uniform: kept in scalar registers), and the
+// live range [begin, end) of that scene's fluid entries.  Returns false for
+// surplus workgroups.
+template <class real>
+__device__ inline bool scene_block(const Scenes<real>& sc, const int* __restrict__ cs, int& s,
+                                   Cst<real>& c, int& first, int& end) {
+  const int b = blockIdx.x;
+  if (b >= sc.blk_off[sc.S]) return false;
+  int lo = 0, hi = sc.S;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (sc.blk_off[mid] <= b) lo = mid;
+    else hi = mid;
+  }
+  s = lo;
+  c = sc.tab[s];
+  first = cs[c.cbase] + (b - sc.blk_off[s]) * blockDim.x;
+  end = cs[c.cbase + c.ncell];
+  return true;
 }
 
 // |q|^2 in the reference's operand order (x, then y, then z).

@@ -7,6 +7,11 @@
 // correctly rounded division and sqrt the results are bit-identical to it.
 // Because x is the fastest key, the three cells jx-1..jx+1 of one (jy, jz)
 // row are contiguous in a sorted set, so a 3-D stencil is 9 ranges.
+//
+// A handle may hold several independent scenes (batched mode); each keeps
+// its own coordinates and constants, and their buckets share one key space.
+// Heavy kernels run scene-uniform workgroups (scene_block); light ones look
+// up the scene of each fluid entry in `sid`.
 #pragma once
 #include "mps_device.h"
 
@@ -65,7 +70,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_reduce(const int* in, int n, in
   if (threadIdx.x == 0) part[blockIdx.x] = total;
 }
 
-// out[i] = sum(in[0..i)) + offset of the tile; out[n] = total when last.
+// out[i] = sum(in[0..i)) + offset of the tile; out[n] = grand total.
 __global__ __launch_bounds__(kBlock) void k_scan_down(const int* in, int n, const int* part_excl,
                                                      int* out) {
   __shared__ int sh[kBlock];
@@ -82,19 +87,57 @@ __global__ __launch_bounds__(kBlock) void k_scan_down(const int* in, int n, cons
     if (base + k < n) out[base + k] = run;
     run += v[k];
   }
-  if (base <= n && n < base + kScanIPT) out[n] = run;  // grand total
+  if (base <= n && n < base + kScanIPT) out[n] = run;
 }
 
-// ---------------------------------------------------------------------------
+// Workgroup table of the scene-uniform launches: blk_off[s] = sum over
+// scenes t < s of ceil(live_t / block).  One workgroup.
+template <class real>
+__global__ __launch_bounds__(kBlock) void k_scene_blocks(const Cst<real>* __restrict__ tab, int S,
+                                                        const int* __restrict__ cs, int block,
+                                                        int* __restrict__ blk_off) {
+  __shared__ int sh[kBlock];
+  int carry = 0;
+  for (int s0 = 0; s0 < S; s0 += kBlock) {
+    const int s = s0 + threadIdx.x;
+    int nb = 0;
+    if (s < S) {
+      const Cst<real>& c = tab[s];
+      const int live = cs[c.cbase + c.ncell] - cs[c.cbase];
+      nb = (live + block - 1) / block;
+    }
+    int total;
+    const int ex = block_exclusive_scan(nb, sh, total);
+    if (s < S) blk_off[s] = carry + ex;
+    carry += total;
+  }
+  if (threadIdx.x == 0) blk_off[S] = carry;
+}
+
+// Scene owning global bucket k (binary search on cbase).
+template <class real>
+__device__ inline int scene_of_cell(const Cst<real>* __restrict__ tab, int S, int k) {
+  int lo = 0, hi = S;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (tab[mid].cbase <= k) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
 // near[k] = 1 when any entry of the set with cell_start cs lies in the
 // 27-bucket (9 in 2-D) stencil of bucket k.  Lets a particle skip a stencil
 // pass over a set that is empty around it.
 template <class real, int DIM>
-__global__ void k_cell_near(const int* __restrict__ cs, Cst<real> c, uint8_t* __restrict__ near) {
+__global__ void k_cell_near(const int* __restrict__ cs, const Cst<real>* __restrict__ tab, int S,
+                            int tot, uint8_t* __restrict__ near) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= c.ncell) return;
-  const int iz = DIM == 3 ? k / c.nbxy : 0;
-  const int rem = k - iz * c.nbxy;
+  if (k >= tot) return;
+  const Cst<real> c = tab[scene_of_cell(tab, S, k)];
+  const int l = k - c.cbase;
+  const int iz = DIM == 3 ? l / c.nbxy : 0;
+  const int rem = l - iz * c.nbxy;
   const int iy = rem / c.nb[0], ix = rem - iy * c.nb[0];
   if (ix < 1 || ix > c.nb[0] - 2 || iy < 1 || iy > c.nb[1] - 2 ||
       (DIM == 3 && (iz < 1 || iz > c.nb[2] - 2))) {
@@ -114,19 +157,28 @@ __global__ void k_cell_near(const int* __restrict__ cs, Cst<real> c, uint8_t* __
 
 template <class real, int DIM>
 __global__ void k_bkt_key(const Vec<real, DIM>* __restrict__ pos, const int8_t* __restrict__ typ,
-                          int n_static, const int* __restrict__ n_dyn, Cst<real> c,
-                          int* __restrict__ key, int* __restrict__ slot, int* __restrict__ cnt) {
-  const int n = n_dyn ? *n_dyn : n_static;
+                          const int* __restrict__ sid, const Cst<real>* __restrict__ tab, int n,
+                          int tot, int* __restrict__ key, int* __restrict__ slot,
+                          int* __restrict__ cnt) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    int k;
-    if (typ && typ[i] == kGST) {
-      k = c.ncell;
-    } else {
+    int k = tot;
+    if (typ[i] != kGST) {
+      const Cst<real> c = tab[sid[i]];
       Cell<real, DIM> q = cell_of<real, DIM>(pos[i], c);
       k = cell_key(q.ix, q.iy, q.iz, c);
     }
     key[i] = k;
     slot[i] = atomicAdd(&cnt[k], 1);
+  }
+}
+
+// Mirror raw entries carry their key; empty slots (key < 0) are skipped.
+__global__ void k_key_count(const int* __restrict__ key, const int* __restrict__ n_dyn, int mult,
+                            int* __restrict__ slot, int* __restrict__ cnt) {
+  const int n = *n_dyn * mult;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int k = key[i];
+    if (k >= 0) slot[i] = atomicAdd(&cnt[k], 1);
   }
 }
 
@@ -142,17 +194,16 @@ __global__ void k_bkt_scatter(const int* __restrict__ key, const int* __restrict
 }
 
 // Final position of an element = start of its cell + number of elements of
-// the same cell with a smaller id.  The deleted tail (key == ncell) keeps the
+// the same cell with a smaller id.  The deleted tail (key == tot) keeps the
 // scatter order: its order is never observed.
 __global__ void k_bkt_rank(const int* __restrict__ tmp, const int* __restrict__ key,
                            const int* __restrict__ id, const int* __restrict__ cs, int n_static,
-                           const int* __restrict__ n_dyn, int mult, int ncell,
-                           int* __restrict__ perm) {
-  const int n = n_dyn ? *n_dyn * mult : n_static;
+                           const int* __restrict__ n_dyn, int tot, int* __restrict__ perm) {
+  const int n = n_dyn ? *n_dyn : n_static;
   for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
     const int i = tmp[s];
     const int k = key[i];
-    if (k == ncell) {
+    if (k == tot) {
       perm[s] = i;
       continue;
     }
@@ -174,6 +225,7 @@ struct FluidArrays {
   real* nd;
   int8_t* typ;
   int* id;
+  int* sid;
 };
 
 template <class real, int DIM>
@@ -189,6 +241,7 @@ __global__ void k_gather_fluid(const int* __restrict__ perm, int n, FluidArrays<
     dst.nd[s] = src.nd[i];
     dst.typ[s] = src.typ[i];
     dst.id[s] = src.id[i];
+    dst.sid[s] = src.sid[i];
   }
 }
 
@@ -207,15 +260,80 @@ __global__ void k_gather_mirror(const int* __restrict__ perm, const int* __restr
   }
 }
 
+// Copies the deleted tail [cs[tot], n) of double-buffered fields (fused
+// kernels only visit live particles); with `s_in` it also adds prs to pav.
+template <class real, int DIM>
+__global__ void k_copy_tail(const int* __restrict__ cs, int tot, int n,
+                            const Vec<real, DIM>* __restrict__ a_in, Vec<real, DIM>* __restrict__ a_out,
+                            const Vec<real, DIM>* __restrict__ b_in, Vec<real, DIM>* __restrict__ b_out,
+                            const real* __restrict__ s_in, real* __restrict__ s_out,
+                            real* __restrict__ pav) {
+  for (int i = cs[tot] + blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    if (a_out) a_out[i] = a_in[i];
+    if (b_out) b_out[i] = b_in[i];
+    if (s_out) {
+      s_out[i] = s_in[i];
+      pav[i] += s_in[i];
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Mirror generation + cull: d_GenMRR_nonslip (kernel.cu:720-893) followed by
-// d_MRRinout (kernel.cu:932-985) for each generated mirror; survivors are
-// appended to the raw mirror set.
+// d_MRRinout (kernel.cu:932-985) for each generated mirror.
+
+// Plane mirror of (px,py,pz) across the plane of wall w with normal n
+// (kernel.cu:815-836); false when the particle lies behind the plane.
+template <class real, int DIM>
+__device__ inline bool plane_mirror(real px, real py, real pz, const Vec<real, DIM>& w,
+                                    const Vec<real, DIM>& n, real& mx, real& my, real& mz) {
+  const real wpx = w.x, wpy = w.y, wpz = vz(w);
+  const real nx = n.x, ny = n.y, nz = vz(n);
+  const real PWx = px - wpx, PWy = py - wpy, PWz = pz - wpz;
+  const real PW_WLL = dot3<real, DIM>(PWx, PWy, PWz, nx, ny, nz);
+  if (PW_WLL < real(0)) return false;
+  const real distance = PW_WLL / sqrt(norm2<real, DIM>(nx, ny, nz));
+  mx = wpx + PWx - real(2) * distance * nx;
+  my = wpy + PWy - real(2) * distance * ny;
+  mz = DIM == 3 ? wpz + PWz - real(2) * distance * nz : real(0);
+  return true;
+}
+
+// Edge mirror from the registered Edge walls edg[0..ne) (kernel.cu:848-877);
+// false when the particle lies behind the synthetic plane.
+template <class real, int DIM>
+__device__ inline bool edge_mirror(real px, real py, real pz, const int* edg, int ne,
+                                   const Vec<real, DIM>* __restrict__ wpos,
+                                   const Vec<real, DIM>* __restrict__ wnrm, real& mx, real& my,
+                                   real& mz) {
+  real sx = 0, sy = 0, sz = 0;
+  for (int k = 0; k < ne; k++) {
+    const Vec<real, DIM> n = wnrm[edg[k]];
+    sx += n.x;
+    sy += n.y;
+    sz += vz(n);
+  }
+  const Vec<real, DIM> w = wpos[edg[ne - 1]];  // posw and PWvec of the last registered Edge wall
+  const real wpx = w.x, wpy = w.y, wpz = vz(w);
+  const real PWx = px - wpx, PWy = py - wpy, PWz = pz - wpz;
+  const real abs_s = sqrt(norm2<real, DIM>(sx, sy, sz));
+  sx /= abs_s;
+  sy /= abs_s;
+  sz /= abs_s;
+  const real PW_WLL = dot3<real, DIM>(PWx, PWy, PWz, sx, sy, sz);
+  if (PW_WLL < real(0)) return false;
+  const real distance = PW_WLL / sqrt(norm2<real, DIM>(sx, sy, sz));
+  mx = wpx + PWx - real(2) * distance * sx;
+  my = wpy + PWy - real(2) * distance * sy;
+  mz = DIM == 3 ? wpz + PWz - real(2) * distance * sz : real(0);
+  return true;
+}
 
 // Nearest wall particle within r of a mirror; the first met wins a tie
 // (kernel.cu:965-971).  A candidate can only pass `dist <= r && dist <
 // walldist` when d2 <= r2_hi and d2 <= d2 of the current best (sqrt is
 // correctly rounded, hence monotone), so sqrt runs only for those.
+// Returns true when the mirror is culled (on the fluid side of that wall).
 template <class real, int DIM>
 __device__ inline bool mirror_culled(real mx, real my, real mz, const Vec<real, DIM>* __restrict__ wpos,
                                      const Vec<real, DIM>* __restrict__ wnrm,
@@ -254,33 +372,10 @@ __device__ inline bool mirror_culled(real mx, real my, real mz, const Vec<real, 
   return found && inout >= real(0);
 }
 
-// Plane mirror of (px,py,pz) across the plane of wall w with normal n
-// (kernel.cu:815-836); false when the particle lies behind the plane.
 template <class real, int DIM>
-__device__ inline bool plane_mirror(real px, real py, real pz, const Vec<real, DIM>& w,
-                                    const Vec<real, DIM>& n, real& mx, real& my, real& mz) {
-  const real wpx = w.x, wpy = w.y, wpz = vz(w);
-  const real nx = n.x, ny = n.y, nz = vz(n);
-  const real PWx = px - wpx, PWy = py - wpy, PWz = pz - wpz;
-  const real PW_WLL = dot3<real, DIM>(PWx, PWy, PWz, nx, ny, nz);
-  if (PW_WLL < real(0)) return false;
-  const real distance = PW_WLL / sqrt(norm2<real, DIM>(nx, ny, nz));
-  mx = wpx + PWx - real(2) * distance * nx;
-  my = wpy + PWy - real(2) * distance * ny;
-  mz = DIM == 3 ? wpz + PWz - real(2) * distance * nz : real(0);
-  return true;
-}
-
-template <class real, int DIM>
-__device__ inline void emit_mirror(real mx, real my, real mz, real prs, int id,
-                                   Vec<real, DIM>* __restrict__ mpos, real* __restrict__ mprs,
-                                   int* __restrict__ mid, int* __restrict__ mcount, int mcap) {
-  const int slot = wave_alloc(mcount);
-  if (slot < mcap) {
-    mpos[slot] = mkvec<real, DIM>(mx, my, mz);
-    mprs[slot] = prs;
-    mid[slot] = id;
-  }
+__device__ inline int mirror_key(real mx, real my, real mz, const Cst<real>& c) {
+  const Cell<real, DIM> q = cell_of<real, DIM>(mkvec<real, DIM>(mx, my, mz), c);
+  return cell_key(q.ix, q.iy, q.iz, c);
 }
 
 // ---------------------------------------------------------------------------
@@ -299,20 +394,29 @@ __device__ inline void emit_mirror(real mx, real my, real mz, real prs, int id,
 //    nearest-wall test (kernel.cu:944-983): the lanes split the candidates,
 //    keep the first minimum of their share, and a wave reduction picks the
 //    smallest (dist, index), which is the reference's first-met minimum.
-//  * slot k of listed particle t goes to raw entry 10*t + k (id -1 if empty).
+//  * slot k of listed particle t goes to raw entry 10*t + k (key -1 if empty).
 
 constexpr int kInf = 0x7fffffff;
 
 template <class real, int DIM>
 __global__ __launch_bounds__(kBlock) void k_mirror_near(const Vec<real, DIM>* __restrict__ fpos,
-                                                       const int* __restrict__ fcs, int n_alloc,
-                                                       const uint8_t* __restrict__ wnear, Cst<real> c,
+                                                       const int* __restrict__ fcs,
+                                                       Scenes<real> sc,
+                                                       const uint8_t* __restrict__ wnear,
                                                        int* __restrict__ near_list,
+                                                       int* __restrict__ near_scene,
                                                        int* __restrict__ near_count) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n_alloc || i >= fcs[c.ncell]) return;
+  int s, first, end;
+  Cst<real> c;
+  if (!scene_block(sc, fcs, s, c, first, end)) return;
+  const int i = first + threadIdx.x;
+  if (i >= end) return;
   const Cell<real, DIM> q = cell_of<real, DIM>(fpos[i], c);
-  if (wnear[cell_key(q.ix, q.iy, q.iz, c)]) near_list[wave_alloc(near_count)] = i;
+  if (wnear[cell_key(q.ix, q.iy, q.iz, c)]) {
+    const int t = wave_alloc(near_count);
+    near_list[t] = i;
+    near_scene[t] = s;
+  }
 }
 
 template <class real>
@@ -361,27 +465,39 @@ __device__ inline BestWall<real> wave_nearest_wall(real mx, real my, real mz,
 }
 
 template <class real, int DIM>
+__device__ inline bool wave_culled(real mx, real my, real mz, const Vec<real, DIM>* __restrict__ wpos,
+                                   const Vec<real, DIM>* __restrict__ wnrm,
+                                   const int* __restrict__ wcs, const Cst<real>& c, int lane) {
+  const BestWall<real> bw = wave_nearest_wall<real, DIM>(mx, my, mz, wpos, wcs, c, lane);
+  if (bw.j < 0) return false;
+  const Vec<real, DIM> w = wpos[bw.j], n = wnrm[bw.j];
+  return dot3<real, DIM>(mx - w.x, my - w.y, mz - vz(w), n.x, n.y, vz(n)) >= real(0);
+}
+
+template <class real, int DIM>
 __global__ __launch_bounds__(64) void k_mirror_wave(
     const Vec<real, DIM>* __restrict__ fpos, const real* __restrict__ fprs,
     const int* __restrict__ fid, const int* __restrict__ near_list,
-    const int* __restrict__ near_count, const Vec<real, DIM>* __restrict__ wpos,
+    const int* __restrict__ near_scene, const int* __restrict__ near_count,
+    const Cst<real>* __restrict__ tab, const Vec<real, DIM>* __restrict__ wpos,
     const Vec<real, DIM>* __restrict__ wnrm, const uint8_t* __restrict__ wcls,
-    const int* __restrict__ wcs, Cst<real> c, Vec<real, DIM>* __restrict__ mpos,
-    real* __restrict__ mprs, int* __restrict__ mid) {
+    const int* __restrict__ wcs, Vec<real, DIM>* __restrict__ mpos, real* __restrict__ mprs,
+    int* __restrict__ mid, int* __restrict__ mkey) {
   __shared__ int first_reg[64], first_edge[64], sorted_reg[kNumMRR], sorted_edge[kNumMRR];
   __shared__ Vec<real, DIM> s_pos[kNumMRR];
   __shared__ real s_prs[kNumMRR];
-  __shared__ int s_id[kNumMRR];
+  __shared__ int s_id[kNumMRR], s_key[kNumMRR];
   const int lane = threadIdx.x;
   const int nnear = *near_count;
   for (int t = blockIdx.x; t < nnear; t += gridDim.x) {
     const int i = near_list[t];
+    const Cst<real> c = tab[near_scene[t]];
     const Vec<real, DIM> p = fpos[i];
     const real px = p.x, py = p.y, pz = vz(p);
     const Cell<real, DIM> q = cell_of<real, DIM>(p, c);
     first_reg[lane] = kInf;
     first_edge[lane] = kInf;
-    if (lane < kNumMRR) s_id[lane] = -1;
+    if (lane < kNumMRR) s_key[lane] = -1;
     __syncthreads();
     const real r2_2 = real(1) * c.r2;
     const int zlo = DIM == 3 ? q.iz - 1 : 0, zhi = DIM == 3 ? q.iz + 1 : 0;
@@ -416,104 +532,59 @@ __global__ __launch_bounds__(64) void k_mirror_wave(
       const int F = sorted_reg[k];
       real mx, my, mz;
       if (!plane_mirror<real, DIM>(px, py, pz, wpos[F], wnrm[F], mx, my, mz)) continue;
-      const BestWall<real> bw = wave_nearest_wall<real, DIM>(mx, my, mz, wpos, wcs, c, lane);
-      bool culled = false;
-      if (bw.j >= 0) {
-        const Vec<real, DIM> w = wpos[bw.j], n = wnrm[bw.j];
-        culled = dot3<real, DIM>(mx - w.x, my - w.y, mz - vz(w), n.x, n.y, vz(n)) >= real(0);
-      }
-      if (!culled && lane == 0) {
+      if (!wave_culled<real, DIM>(mx, my, mz, wpos, wnrm, wcs, c, lane) && lane == 0) {
         s_pos[k] = mkvec<real, DIM>(mx, my, mz);
         s_prs[k] = prs_i;
         s_id[k] = base_id + k;
+        s_key[k] = mirror_key<real, DIM>(mx, my, mz, c);
       }
     }
     if (ne >= 2 && nreg < kNumMRR) {  // edge mirror, kernel.cu:840-888
-      real sx = 0, sy = 0, sz = 0;
-      for (int k = 0; k < ne; k++) {
-        const Vec<real, DIM> n = wnrm[sorted_edge[k]];
-        sx += n.x;
-        sy += n.y;
-        sz += vz(n);
-      }
-      const Vec<real, DIM> w = wpos[sorted_edge[ne - 1]];
-      const real wpx = w.x, wpy = w.y, wpz = vz(w);
-      const real PWx = px - wpx, PWy = py - wpy, PWz = pz - wpz;
-      const real abs_s = sqrt(norm2<real, DIM>(sx, sy, sz));
-      sx /= abs_s;
-      sy /= abs_s;
-      sz /= abs_s;
-      const real PW_WLL = dot3<real, DIM>(PWx, PWy, PWz, sx, sy, sz);
-      if (PW_WLL >= real(0)) {
-        const real distance = PW_WLL / sqrt(norm2<real, DIM>(sx, sy, sz));
-        const real mx = wpx + PWx - real(2) * distance * sx;
-        const real my = wpy + PWy - real(2) * distance * sy;
-        const real mz = DIM == 3 ? wpz + PWz - real(2) * distance * sz : real(0);
-        const BestWall<real> bw = wave_nearest_wall<real, DIM>(mx, my, mz, wpos, wcs, c, lane);
-        bool culled = false;
-        if (bw.j >= 0) {
-          const Vec<real, DIM> w2 = wpos[bw.j], n = wnrm[bw.j];
-          culled = dot3<real, DIM>(mx - w2.x, my - w2.y, mz - vz(w2), n.x, n.y, vz(n)) >= real(0);
-        }
-        if (!culled && lane == 0) {
-          s_pos[nreg] = mkvec<real, DIM>(mx, my, mz);
-          s_prs[nreg] = prs_i;
-          s_id[nreg] = base_id + nreg;
-        }
+      real mx, my, mz;
+      if (edge_mirror<real, DIM>(px, py, pz, sorted_edge, ne, wpos, wnrm, mx, my, mz) &&
+          !wave_culled<real, DIM>(mx, my, mz, wpos, wnrm, wcs, c, lane) && lane == 0) {
+        s_pos[nreg] = mkvec<real, DIM>(mx, my, mz);
+        s_prs[nreg] = prs_i;
+        s_id[nreg] = base_id + nreg;
+        s_key[nreg] = mirror_key<real, DIM>(mx, my, mz, c);
       }
     }
     __syncthreads();
     if (lane < kNumMRR) {
       const size_t o = (size_t)t * kNumMRR + lane;
-      mid[o] = s_id[lane];
-      if (s_id[lane] >= 0) {
+      mkey[o] = s_key[lane];
+      if (s_key[lane] >= 0) {
         mpos[o] = s_pos[lane];
         mprs[o] = s_prs[lane];
+        mid[o] = s_id[lane];
       }
     }
     __syncthreads();
   }
 }
 
-// Bucket key of a raw mirror slot; empty slots (id < 0) get key -1 and are
-// neither counted nor scattered.
-template <class real, int DIM>
-__global__ void k_mirror_key(const Vec<real, DIM>* __restrict__ pos, const int* __restrict__ id,
-                             const int* __restrict__ n_dyn, int mult, Cst<real> c,
-                             int* __restrict__ key, int* __restrict__ slot, int* __restrict__ cnt) {
-  const int n = *n_dyn * mult;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    if (id[i] < 0) {
-      key[i] = -1;
-      continue;
-    }
-    Cell<real, DIM> q = cell_of<real, DIM>(pos[i], c);
-    const int k = cell_key(q.ix, q.iy, q.iz, c);
-    key[i] = k;
-    slot[i] = atomicAdd(&cnt[k], 1);
-  }
-}
-
-// General mirror generation (any number of distinct wall normals).
+// General mirror generation (any number of distinct wall normals), one lane
+// per particle, emitting compacted raw entries.
 template <class real, int DIM>
 __global__ __launch_bounds__(kBlock) void k_mirror_gen(
     const Vec<real, DIM>* __restrict__ fpos, const real* __restrict__ fprs,
-    const int* __restrict__ fid, const int* __restrict__ fcs, int n_alloc,
+    const int* __restrict__ fid, const int* __restrict__ fcs, Scenes<real> sc,
     const Vec<real, DIM>* __restrict__ wpos, const Vec<real, DIM>* __restrict__ wnrm,
-    const int8_t* __restrict__ wedge, const int* __restrict__ wcs, Cst<real> c,
+    const int8_t* __restrict__ wedge, const int* __restrict__ wcs,
     Vec<real, DIM>* __restrict__ mpos, real* __restrict__ mprs, int* __restrict__ mid,
-    int* __restrict__ mcount, int mcap) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n_alloc) return;
-  const int n_live = fcs[c.ncell];
-  if (i >= n_live) return;
+    int* __restrict__ mkey, int* __restrict__ mcount, int mcap) {
+  int s, first, end;
+  Cst<real> c;
+  if (!scene_block(sc, fcs, s, c, first, end)) return;
+  const int i = first + threadIdx.x;
+  if (i >= end) return;
   const Vec<real, DIM> p = fpos[i];
   const real px = p.x, py = p.y, pz = vz(p);
   const Cell<real, DIM> q = cell_of<real, DIM>(p, c);
   const real r2_2 = real(1) * c.r2;
   int reg[kNumMRR], edg[kNumMRR];
   int nreg = 0, ne = 0;
-  bool wexist = false, eexist = false;
+  bool wexist = false;
   const int zlo = DIM == 3 ? q.iz - 1 : 0, zhi = DIM == 3 ? q.iz + 1 : 0;
   for (int jz = zlo; jz <= zhi; jz++)
     for (int jy = q.iy - 1; jy <= q.iy + 1; jy++) {
@@ -531,7 +602,6 @@ __global__ __launch_bounds__(kBlock) void k_mirror_gen(
           }
           if (!dup && nreg < kNumMRR) reg[nreg++] = j;
           if (wedge[j] == kEdge) {
-            eexist = true;
             bool edup = false;
             for (int k = 0; k < ne; k++) {
               const Vec<real, DIM> nk = wnrm[edg[k]];
@@ -545,58 +615,25 @@ __global__ __launch_bounds__(kBlock) void k_mirror_gen(
   if (!wexist) return;
   const real prs_i = fprs[i];
   const int base_id = fid[i] * kNumMRR;
+  auto emit = [&](real mx, real my, real mz, int slot_id) {
+    const int slot = wave_alloc(mcount);
+    if (slot < mcap) {
+      mpos[slot] = mkvec<real, DIM>(mx, my, mz);
+      mprs[slot] = prs_i;
+      mid[slot] = slot_id;
+      mkey[slot] = mirror_key<real, DIM>(mx, my, mz, c);
+    }
+  };
   for (int k = 0; k < nreg; k++) {  // plane mirrors, kernel.cu:811-838
-    const Vec<real, DIM> w = wpos[reg[k]];
-    const Vec<real, DIM> n = wnrm[reg[k]];
-    const real wpx = w.x, wpy = w.y, wpz = vz(w);
-    const real nx = n.x, ny = n.y, nz = vz(n);
-    const real PWx = px - wpx, PWy = py - wpy, PWz = pz - wpz;
-    const real PW_WLL = dot3<real, DIM>(PWx, PWy, PWz, nx, ny, nz);
-    if (PW_WLL < real(0)) continue;
-    const real distance = PW_WLL / sqrt(norm2<real, DIM>(nx, ny, nz));
-    const real mx = wpx + PWx - real(2) * distance * nx;
-    const real my = wpy + PWy - real(2) * distance * ny;
-    const real mz = DIM == 3 ? wpz + PWz - real(2) * distance * nz : real(0);
-    if (mirror_culled<real, DIM>(mx, my, mz, wpos, wnrm, wcs, c)) continue;
-    const int slot = wave_alloc(mcount);
-    if (slot < mcap) {
-      mpos[slot] = mkvec<real, DIM>(mx, my, mz);
-      mprs[slot] = prs_i;
-      mid[slot] = base_id + k;
-    }
+    real mx, my, mz;
+    if (!plane_mirror<real, DIM>(px, py, pz, wpos[reg[k]], wnrm[reg[k]], mx, my, mz)) continue;
+    if (!mirror_culled<real, DIM>(mx, my, mz, wpos, wnrm, wcs, c)) emit(mx, my, mz, base_id + k);
   }
-  if (eexist && ne >= 2 && nreg < kNumMRR) {  // edge mirror, kernel.cu:840-888
-    real wpx = 0, wpy = 0, wpz = 0, sx = 0, sy = 0, sz = 0, PWx = 0, PWy = 0, PWz = 0;
-    for (int k = 0; k < ne; k++) {
-      const Vec<real, DIM> w = wpos[edg[k]];
-      const Vec<real, DIM> n = wnrm[edg[k]];
-      wpx = w.x;
-      wpy = w.y;
-      wpz = vz(w);
-      sx += n.x;
-      sy += n.y;
-      sz += vz(n);
-      PWx = px - wpx;
-      PWy = py - wpy;
-      PWz = pz - wpz;
-    }
-    const real abs_s = sqrt(norm2<real, DIM>(sx, sy, sz));
-    sx /= abs_s;
-    sy /= abs_s;
-    sz /= abs_s;
-    const real PW_WLL = dot3<real, DIM>(PWx, PWy, PWz, sx, sy, sz);
-    if (PW_WLL < real(0)) return;
-    const real distance = PW_WLL / sqrt(norm2<real, DIM>(sx, sy, sz));
-    const real mx = wpx + PWx - real(2) * distance * sx;
-    const real my = wpy + PWy - real(2) * distance * sy;
-    const real mz = DIM == 3 ? wpz + PWz - real(2) * distance * sz : real(0);
-    if (mirror_culled<real, DIM>(mx, my, mz, wpos, wnrm, wcs, c)) return;
-    const int slot = wave_alloc(mcount);
-    if (slot < mcap) {
-      mpos[slot] = mkvec<real, DIM>(mx, my, mz);
-      mprs[slot] = prs_i;
-      mid[slot] = base_id + nreg;
-    }
+  if (ne >= 2 && nreg < kNumMRR) {  // edge mirror, kernel.cu:840-888
+    real mx, my, mz;
+    if (edge_mirror<real, DIM>(px, py, pz, edg, ne, wpos, wnrm, mx, my, mz) &&
+        !mirror_culled<real, DIM>(mx, my, mz, wpos, wnrm, wcs, c))
+      emit(mx, my, mz, base_id + nreg);
   }
 }
 
@@ -645,24 +682,19 @@ __device__ inline void for_hits(int b, int e, T&& test, Body&& body) {
 
 // ---------------------------------------------------------------------------
 // d_VscTrm (kernel.cu:1013-1119), optionally fused with d_UpPcl1
-// (kernel.cu:1138-1149).
+// (kernel.cu:1138-1149).  Scene-uniform workgroups over live particles.
 template <class real, int DIM, bool FUSE>
 __global__ __launch_bounds__(kBlock) void k_vsc(
     const Vec<real, DIM>* __restrict__ pos, const Vec<real, DIM>* __restrict__ vel,
-    const int* __restrict__ fcs, int n_alloc, const Vec<real, DIM>* __restrict__ mpos,
-    const int* __restrict__ mcs, const uint8_t* __restrict__ mnear, Cst<real> c,
+    const int* __restrict__ fcs, Scenes<real> sc, const Vec<real, DIM>* __restrict__ mpos,
+    const int* __restrict__ mcs, const uint8_t* __restrict__ mnear,
     Vec<real, DIM>* __restrict__ acc_out, Vec<real, DIM>* __restrict__ pos_out,
     Vec<real, DIM>* __restrict__ vel_out, int8_t* __restrict__ typ, real* __restrict__ prs) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n_alloc) return;
-  const int n_live = fcs[c.ncell];
-  if (i >= n_live) {
-    if constexpr (FUSE) {
-      pos_out[i] = pos[i];
-      vel_out[i] = vel[i];
-    }
-    return;
-  }
+  int s, first, end;
+  Cst<real> c;
+  if (!scene_block(sc, fcs, s, c, first, end)) return;
+  const int i = first + threadIdx.x;
+  if (i >= end) return;
   const Vec<real, DIM> p = pos[i], v = vel[i];
   const real px = p.x, py = p.y, pz = vz(p);
   const real vx = v.x, vy = v.y, vvz = vz(v);
@@ -693,26 +725,26 @@ __global__ __launch_bounds__(kBlock) void k_vsc(
           });
     }
   if (mnear[cell_key(q.ix, q.iy, q.iz, c)])
-  for (int jz = zlo; jz <= zhi; jz++)
-    for (int jy = q.iy - 1; jy <= q.iy + 1; jy++) {
-      const int b = mcs[cell_key(q.ix - 1, jy, jz, c)], e = mcs[cell_key(q.ix + 1, jy, jz, c) + 1];
-      for_hits(
-          b, e,
-          [&](int j) {
-            const Vec<real, DIM> pj = mpos[j];
-            return norm2<real, DIM>(pj.x - px, pj.y - py, vz(pj) - pz) <= r2_hi;
-          },
-          [&](int j) {
-            const Vec<real, DIM> pj = mpos[j];
-            const real qx = pj.x - px, qy = pj.y - py, qz = vz(pj) - pz;
-            const real dist = sqrt(norm2<real, DIM>(qx, qy, qz));
-            if (dist <= r) {
-              ax += real(2) * (real(0) - vx) * l0 / dist / dist / dist;
-              ay += real(2) * (real(0) - vy) * l0 / dist / dist / dist;
-              if constexpr (DIM == 3) az += real(2) * (real(0) - vvz) * l0 / dist / dist / dist;
-            }
-          });
-    }
+    for (int jz = zlo; jz <= zhi; jz++)
+      for (int jy = q.iy - 1; jy <= q.iy + 1; jy++) {
+        const int b = mcs[cell_key(q.ix - 1, jy, jz, c)], e = mcs[cell_key(q.ix + 1, jy, jz, c) + 1];
+        for_hits(
+            b, e,
+            [&](int j) {
+              const Vec<real, DIM> pj = mpos[j];
+              return norm2<real, DIM>(pj.x - px, pj.y - py, vz(pj) - pz) <= r2_hi;
+            },
+            [&](int j) {
+              const Vec<real, DIM> pj = mpos[j];
+              const real qx = pj.x - px, qy = pj.y - py, qz = vz(pj) - pz;
+              const real dist = sqrt(norm2<real, DIM>(qx, qy, qz));
+              if (dist <= r) {
+                ax += real(2) * (real(0) - vx) * l0 / dist / dist / dist;
+                ay += real(2) * (real(0) - vy) * l0 / dist / dist / dist;
+                if constexpr (DIM == 3) az += real(2) * (real(0) - vvz) * l0 / dist / dist / dist;
+              }
+            });
+      }
   const real accx = c.n0knm * ax + c.g[0];
   const real accy = c.n0knm * ay + c.g[1];
   const real accz = DIM == 3 ? c.n0knm * az + c.g[2] : real(0);
@@ -734,9 +766,11 @@ __global__ __launch_bounds__(kBlock) void k_vsc(
 template <class real, int DIM>
 __global__ void k_up1(Vec<real, DIM>* __restrict__ pos, Vec<real, DIM>* __restrict__ vel,
                       Vec<real, DIM>* __restrict__ acc, int8_t* __restrict__ typ,
-                      real* __restrict__ prs, int n, Cst<real> c) {
+                      real* __restrict__ prs, const int* __restrict__ sid,
+                      const Cst<real>* __restrict__ tab, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n || typ[i] != kFLD) return;
+  const Cst<real> c = tab[sid[i]];
   const Vec<real, DIM> a = acc[i], p = pos[i], v = vel[i];
   real nvx = v.x + a.x * c.dt, nvy = v.y + a.y * c.dt, nvz = DIM == 3 ? vz(v) + vz(a) * c.dt : real(0);
   real nx = p.x + nvx * c.dt, ny = p.y + nvy * c.dt, nz = DIM == 3 ? vz(p) + nvz * c.dt : real(0);
@@ -749,268 +783,16 @@ __global__ void k_up1(Vec<real, DIM>* __restrict__ pos, Vec<real, DIM>* __restri
   vel[i] = mkvec<real, DIM>(nvx, nvy, nvz);
 }
 
-// ---------------------------------------------------------------------------
-// d_ChkCol + d_UpChkCol (kernel.cu:1168-1287).  Jacobi: reads vel, writes
-// vel_out (and acc_out in phase mode).  Inside each cell the fluid
-// particles come before the wall particles, as in the restated lists.
-template <class real, int DIM, bool PHASE>
-__global__ __launch_bounds__(kBlock) void k_col(
-    const Vec<real, DIM>* __restrict__ pos, const Vec<real, DIM>* __restrict__ vel,
-    const int* __restrict__ fcs, int n_alloc, const Vec<real, DIM>* __restrict__ wpos,
-    const Vec<real, DIM>* __restrict__ wvel, const int* __restrict__ wcs,
-    const Vec<real, DIM>* __restrict__ mpos, const int* __restrict__ mcs, Cst<real> c,
-    Vec<real, DIM>* __restrict__ vel_out, Vec<real, DIM>* __restrict__ acc_out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n_alloc) return;
-  const int n_live = fcs[c.ncell];
-  if (i >= n_live) {
-    vel_out[i] = vel[i];
-    return;
-  }
-  const Vec<real, DIM> p = pos[i], v = vel[i];
-  const real px = p.x, py = p.y, pz = vz(p);
-  const real vx = v.x, vy = v.y, vvz = vz(v);
-  real v2x = vx, v2y = vy, v2z = vvz;
-  const real mi = c.mass;
-  const Cell<real, DIM> q = cell_of<real, DIM>(p, c);
-  const int zlo = DIM == 3 ? q.iz - 1 : 0, zhi = DIM == 3 ? q.iz + 1 : 0;
-  for (int jz = zlo; jz <= zhi; jz++)
-    for (int jy = q.iy - 1; jy <= q.iy + 1; jy++)
-      for (int jx = q.ix - 1; jx <= q.ix + 1; jx++) {
-        const int ck = cell_key(jx, jy, jz, c);
-        for (int pass = 0; pass < 2; pass++) {
-          const int b = pass == 0 ? fcs[ck] : wcs[ck];
-          const int e = pass == 0 ? fcs[ck + 1] : wcs[ck + 1];
-          const Vec<real, DIM>* __restrict__ P = pass == 0 ? pos : wpos;
-          const Vec<real, DIM>* __restrict__ V = pass == 0 ? vel : wvel;
-          const real lim2 = pass == 0 ? c.rlim2 : c.rlim_wall2;
-          const int self = pass == 0 ? i : -1;
-          for_hits(
-              b, e,
-              [&](int j) {
-                const Vec<real, DIM> pj = P[j];
-                return j != self && norm2<real, DIM>(pj.x - px, pj.y - py, vz(pj) - pz) < lim2;
-              },
-              [&](int j) {
-                const Vec<real, DIM> pj = P[j];
-                const real qx = pj.x - px, qy = pj.y - py, qz = vz(pj) - pz;
-                const real dist2 = norm2<real, DIM>(qx, qy, qz);
-                const Vec<real, DIM> vj = V[j];
-                real fDT = dot3<real, DIM>(vx - vj.x, vy - vj.y, vvz - vz(vj), qx, qy, qz);
-                if (fDT > real(0)) {
-                  const real mj = c.mass;
-                  fDT *= c.col * mj / (mi + mj) / dist2;
-                  v2x -= qx * fDT;
-                  v2y -= qy * fDT;
-                  if constexpr (DIM == 3) v2z -= qz * fDT;
-                }
-              });
-        }
-      }
-  const real mj = c.mass;
-  for (int jz = zlo; jz <= zhi; jz++)
-    for (int jy = q.iy - 1; jy <= q.iy + 1; jy++) {
-      const int b = mcs[cell_key(q.ix - 1, jy, jz, c)], e = mcs[cell_key(q.ix + 1, jy, jz, c) + 1];
-      for_hits(
-          b, e,
-          [&](int j) {
-            const Vec<real, DIM> pj = mpos[j];
-            return norm2<real, DIM>(pj.x - px, pj.y - py, vz(pj) - pz) < c.rlim2;
-          },
-          [&](int j) {
-            const Vec<real, DIM> pj = mpos[j];
-            const real qx = pj.x - px, qy = pj.y - py, qz = vz(pj) - pz;
-            const real dist2 = norm2<real, DIM>(qx, qy, qz);
-            real fDT = dot3<real, DIM>(vx - real(0), vy - real(0), vvz - real(0), qx, qy, qz);
-            if (fDT > real(0)) {
-              fDT *= c.col * mj / (mi + mj) / dist2;
-              v2x -= qx * fDT;
-              v2y -= qy * fDT;
-              if constexpr (DIM == 3) v2z -= qz * fDT;
-            }
-          });
-    }
-  const Vec<real, DIM> out = mkvec<real, DIM>(v2x, v2y, v2z);
-  vel_out[i] = out;
-  if constexpr (PHASE) acc_out[i] = out;
-}
-
-// ---------------------------------------------------------------------------
-// d_MkPrs (kernel.cu:1306-1411).  Writes prs and the number density nd.
-template <class real, int DIM>
-__global__ __launch_bounds__(kBlock) void k_prs(const Vec<real, DIM>* __restrict__ pos,
-                                                 const int* __restrict__ fcs, int n_alloc,
-                                                 const Vec<real, DIM>* __restrict__ mpos,
-                                                 const int* __restrict__ mcs, Cst<real> c,
-                                                 real* __restrict__ prs, real* __restrict__ nd) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n_alloc) return;
-  const int n_live = fcs[c.ncell];
-  if (i >= n_live) return;
-  const Vec<real, DIM> p = pos[i];
-  const real px = p.x, py = p.y, pz = vz(p);
-  const real rp = c.rp;
-  real ni = 0;
-  const Cell<real, DIM> q = cell_of<real, DIM>(p, c);
-  const int zlo = DIM == 3 ? q.iz - 1 : 0, zhi = DIM == 3 ? q.iz + 1 : 0;
-  for (int jz = zlo; jz <= zhi; jz++)
-    for (int jy = q.iy - 1; jy <= q.iy + 1; jy++) {
-      const int b = fcs[cell_key(q.ix - 1, jy, jz, c)], e = fcs[cell_key(q.ix + 1, jy, jz, c) + 1];
-      for_hits(
-          b, e,
-          [&](int j) {
-            const Vec<real, DIM> pj = pos[j];
-            return j != i && norm2<real, DIM>(pj.x - px, pj.y - py, vz(pj) - pz) <= c.rp2_hi;
-          },
-          [&](int j) {
-            const Vec<real, DIM> pj = pos[j];
-            const real dist = sqrt(norm2<real, DIM>(pj.x - px, pj.y - py, vz(pj) - pz));
-            if (dist < rp) ni += (rp / dist) - (dist / rp);
-          });
-    }
-  for (int jz = zlo; jz <= zhi; jz++)
-    for (int jy = q.iy - 1; jy <= q.iy + 1; jy++) {
-      const int b = mcs[cell_key(q.ix - 1, jy, jz, c)], e = mcs[cell_key(q.ix + 1, jy, jz, c) + 1];
-      for_hits(
-          b, e,
-          [&](int j) {
-            const Vec<real, DIM> pj = mpos[j];
-            return norm2<real, DIM>(pj.x - px, pj.y - py, vz(pj) - pz) <= c.rp2_hi;
-          },
-          [&](int j) {
-            const Vec<real, DIM> pj = mpos[j];
-            const real dist = sqrt(norm2<real, DIM>(pj.x - px, pj.y - py, vz(pj) - pz));
-            if (dist < rp) ni += (rp / dist) - (dist / rp);
-          });
-    }
-  real pressure = real(ni > c.n0_grad) * (ni - c.n0_grad) * c.prs_coef * c.mass;
-  if (pressure <= c.pmax && pressure >= real(0)) {
-  } else {
-    if (pressure > c.pmax) pressure = c.pmax;
-    else pressure = real(0);
-  }
-  prs[i] = pressure;
-  nd[i] = ni;
-}
-
-// ---------------------------------------------------------------------------
-// d_PrsGrdTrm, active version (kernel.cu:1651-1794), optionally fused with
-// d_UpPcl2 (kernel.cu:1815-1829).  The fused form writes pos/vel/prs to the
-// other buffer (neighbours still read the old prs) and adds prs to pav.
-template <class real, int DIM, bool FUSE>
-__global__ __launch_bounds__(kBlock) void k_grad(
-    const Vec<real, DIM>* __restrict__ pos, const Vec<real, DIM>* __restrict__ vel,
-    const real* __restrict__ prs, const int* __restrict__ fcs, int n_alloc,
-    const Vec<real, DIM>* __restrict__ mpos, const real* __restrict__ mprs,
-    const int* __restrict__ mcs, Cst<real> c, Vec<real, DIM>* __restrict__ acc_out,
-    Vec<real, DIM>* __restrict__ pos_out, Vec<real, DIM>* __restrict__ vel_out,
-    real* __restrict__ prs_out, real* __restrict__ pav, int8_t* __restrict__ typ) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n_alloc) return;
-  const int n_live = fcs[c.ncell];
-  if (i >= n_live) {
-    if constexpr (FUSE) {
-      pos_out[i] = pos[i];
-      vel_out[i] = vel[i];
-      prs_out[i] = prs[i];
-      pav[i] += prs[i];
-    }
-    return;
-  }
-  const Vec<real, DIM> p = pos[i];
-  const real px = p.x, py = p.y, pz = vz(p);
-  const real rp = c.rp, rp2 = c.rp2;
-  const real prs_i = prs[i];
-  real Prs_min = prs_i;
-  real ax = 0, ay = 0, az = 0;
-  const Cell<real, DIM> q = cell_of<real, DIM>(p, c);
-  const int zlo = DIM == 3 ? q.iz - 1 : 0, zhi = DIM == 3 ? q.iz + 1 : 0;
-  for (int jz = zlo; jz <= zhi; jz++)
-    for (int jy = q.iy - 1; jy <= q.iy + 1; jy++) {
-      const int b = fcs[cell_key(q.ix - 1, jy, jz, c)], e = fcs[cell_key(q.ix + 1, jy, jz, c) + 1];
-      for (int j = b; j < e; j++) {
-        if (j == i) continue;
-        const Vec<real, DIM> pj = pos[j];
-        const real dist2 = norm2<real, DIM>(pj.x - px, pj.y - py, vz(pj) - pz);
-        if (dist2 < rp2) {
-          const real pr = prs[j];
-          if (pr < Prs_min) Prs_min = pr;
-        }
-      }
-    }
-  for (int jz = zlo; jz <= zhi; jz++)
-    for (int jy = q.iy - 1; jy <= q.iy + 1; jy++) {
-      const int b = fcs[cell_key(q.ix - 1, jy, jz, c)], e = fcs[cell_key(q.ix + 1, jy, jz, c) + 1];
-      for_hits(
-          b, e,
-          [&](int j) {
-            const Vec<real, DIM> pj = pos[j];
-            return j != i && norm2<real, DIM>(pj.x - px, pj.y - py, vz(pj) - pz) < rp2;
-          },
-          [&](int j) {
-            const Vec<real, DIM> pj = pos[j];
-            const real qx = pj.x - px, qy = pj.y - py, qz = vz(pj) - pz;
-            const real dist2 = norm2<real, DIM>(qx, qy, qz);
-            const real dist = sqrt(dist2);
-            const real w = ((rp / dist) - (dist / rp)) / dist2;
-            const real d = prs[j] - Prs_min;
-            ax += d * w * qx;
-            ay += d * w * qy;
-            if constexpr (DIM == 3) az += d * w * qz;
-          });
-    }
-  for (int jz = zlo; jz <= zhi; jz++)
-    for (int jy = q.iy - 1; jy <= q.iy + 1; jy++) {
-      const int b = mcs[cell_key(q.ix - 1, jy, jz, c)], e = mcs[cell_key(q.ix + 1, jy, jz, c) + 1];
-      for_hits(
-          b, e,
-          [&](int j) {
-            const Vec<real, DIM> pj = mpos[j];
-            return norm2<real, DIM>(pj.x - px, pj.y - py, vz(pj) - pz) < rp2;
-          },
-          [&](int j) {
-            const Vec<real, DIM> pj = mpos[j];
-            const real qx = pj.x - px, qy = pj.y - py, qz = vz(pj) - pz;
-            const real dist2 = norm2<real, DIM>(qx, qy, qz);
-            const real dist = sqrt(dist2);
-            const real w = ((rp / dist) - (dist / rp)) / dist2;
-            const real d = mprs[j] - Prs_min;
-            ax += d * w * qx;
-            ay += d * w * qy;
-            if constexpr (DIM == 3) az += d * w * qz;
-          });
-    }
-  const real accx = c.invro * ax * c.grad_a;
-  const real accy = c.invro * ay * c.grad_a;
-  const real accz = DIM == 3 ? c.invro * az * c.grad_a : real(0);
-  if constexpr (!FUSE) {
-    acc_out[i] = mkvec<real, DIM>(accx, accy, accz);
-  } else {
-    const Vec<real, DIM> v = vel[i];
-    real nvx = v.x + accx * c.dt, nvy = v.y + accy * c.dt;
-    real nvz = DIM == 3 ? vz(v) + accz * c.dt : real(0);
-    real nx = px + accx * c.dt * c.dt, ny = py + accy * c.dt * c.dt;
-    real nz = DIM == 3 ? pz + accz * c.dt * c.dt : real(0);
-    pav[i] += prs_i;
-    real po = prs_i;
-    if (!chk_pcl<real, DIM>(nx, ny, nz, nvx, nvy, nvz, c)) {
-      typ[i] = kGST;
-      po = real(0);
-    }
-    pos_out[i] = mkvec<real, DIM>(nx, ny, nz);
-    vel_out[i] = mkvec<real, DIM>(nvx, nvy, nvz);
-    prs_out[i] = po;
-  }
-}
-
 // d_UpPcl2 alone (phase mode).
 template <class real, int DIM>
 __global__ void k_up2(Vec<real, DIM>* __restrict__ pos, Vec<real, DIM>* __restrict__ vel,
                       Vec<real, DIM>* __restrict__ acc, int8_t* __restrict__ typ,
-                      real* __restrict__ prs, real* __restrict__ pav, int n, Cst<real> c) {
+                      real* __restrict__ prs, real* __restrict__ pav, const int* __restrict__ sid,
+                      const Cst<real>* __restrict__ tab, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   if (typ[i] == kFLD) {
+    const Cst<real> c = tab[sid[i]];
     const Vec<real, DIM> a = acc[i], p = pos[i], v = vel[i];
     real nvx = v.x + a.x * c.dt, nvy = v.y + a.y * c.dt;
     real nvz = DIM == 3 ? vz(v) + vz(a) * c.dt : real(0);

@@ -20,10 +20,13 @@ constexpr int kNList = 64;
 template <class real, int DIM>
 __global__ __launch_bounds__(kBlock) void k_nlist_build(const Vec<real, DIM>* __restrict__ pos,
                                                        const int* __restrict__ fcs, int n_alloc,
-                                                       Cst<real> c, int* __restrict__ list,
+                                                       Scenes<real> sc, int* __restrict__ list,
                                                        int* __restrict__ count) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n_alloc || i >= fcs[c.ncell]) return;
+  int s, first, end;
+  Cst<real> c;
+  if (!scene_block(sc, fcs, s, c, first, end)) return;
+  const int i = first + threadIdx.x;
+  if (i >= end) return;
   const Vec<real, DIM> p = pos[i];
   const real px = p.x, py = p.y, pz = vz(p), lim = c.rp2_hi;
   const Cell<real, DIM> q = cell_of<real, DIM>(p, c);
@@ -77,14 +80,13 @@ __global__ __launch_bounds__(kBlock) void k_col_list(
     const int* __restrict__ count, const Vec<real, DIM>* __restrict__ wpos,
     const Vec<real, DIM>* __restrict__ wvel, const int* __restrict__ wcs,
     const Vec<real, DIM>* __restrict__ mpos, const int* __restrict__ mcs,
-    const uint8_t* __restrict__ wnear, const uint8_t* __restrict__ mnear, Cst<real> c,
+    const uint8_t* __restrict__ wnear, const uint8_t* __restrict__ mnear, Scenes<real> sc,
     Vec<real, DIM>* __restrict__ vel_out, Vec<real, DIM>* __restrict__ acc_out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n_alloc) return;
-  if (i >= fcs[c.ncell]) {
-    vel_out[i] = vel[i];
-    return;
-  }
+  int s, first, end;
+  Cst<real> c;
+  if (!scene_block(sc, fcs, s, c, first, end)) return;
+  const int i = first + threadIdx.x;
+  if (i >= end) return;
   const Vec<real, DIM> p = pos[i], v = vel[i];
   const real px = p.x, py = p.y, pz = vz(p);
   const real vx = v.x, vy = v.y, vvz = vz(v);
@@ -151,10 +153,13 @@ __global__ __launch_bounds__(kBlock) void k_prs_list(
     const Vec<real, DIM>* __restrict__ pos, const int* __restrict__ fcs, int n_alloc,
     const int* __restrict__ list, const int* __restrict__ count,
     const Vec<real, DIM>* __restrict__ mpos, const int* __restrict__ mcs,
-    const uint8_t* __restrict__ mnear, Cst<real> c, real* __restrict__ prs,
+    const uint8_t* __restrict__ mnear, Scenes<real> sc, real* __restrict__ prs,
     real* __restrict__ nd) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n_alloc || i >= fcs[c.ncell]) return;
+  int s, first, end;
+  Cst<real> c;
+  if (!scene_block(sc, fcs, s, c, first, end)) return;
+  const int i = first + threadIdx.x;
+  if (i >= end) return;
   const Vec<real, DIM> p = pos[i];
   const real px = p.x, py = p.y, pz = vz(p), rp = c.rp, lim = c.rp2_hi;
   real ni = 0;
@@ -191,21 +196,15 @@ __global__ __launch_bounds__(kBlock) void k_grad_list(
     const real* __restrict__ prs, const int* __restrict__ fcs, int n_alloc,
     const int* __restrict__ list, const int* __restrict__ count,
     const Vec<real, DIM>* __restrict__ mpos, const real* __restrict__ mprs,
-    const int* __restrict__ mcs, const uint8_t* __restrict__ mnear, Cst<real> c,
+    const int* __restrict__ mcs, const uint8_t* __restrict__ mnear, Scenes<real> sc,
     Vec<real, DIM>* __restrict__ acc_out, Vec<real, DIM>* __restrict__ pos_out,
     Vec<real, DIM>* __restrict__ vel_out, real* __restrict__ prs_out, real* __restrict__ pav,
     int8_t* __restrict__ typ) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n_alloc) return;
-  if (i >= fcs[c.ncell]) {
-    if constexpr (FUSE) {
-      pos_out[i] = pos[i];
-      vel_out[i] = vel[i];
-      prs_out[i] = prs[i];
-      pav[i] += prs[i];
-    }
-    return;
-  }
+  int s, first, end;
+  Cst<real> c;
+  if (!scene_block(sc, fcs, s, c, first, end)) return;
+  const int i = first + threadIdx.x;
+  if (i >= end) return;
   const Vec<real, DIM> p = pos[i];
   const real px = p.x, py = p.y, pz = vz(p), rp = c.rp, rp2 = c.rp2;
   const real prs_i = prs[i];

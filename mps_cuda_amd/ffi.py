@@ -92,8 +92,13 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.mps_last_error.restype = C.c_char_p
     for fn in ("mps_scene_dambreak_count", "mps_scene_dambreak_fill", "mps_params_default",
                "mps_compute_constants", "mps_compute_edge", "mps_probe_arith", "mps_sync",
-               "mps_set_stream", "mps_timing_enable", "mps_timing_read"):
+               "mps_set_stream", "mps_timing_enable", "mps_timing_read", "mps_create_batch",
+               "mps_get_scene_constants", "mps_set_graph", "mps_set_launch"):
         getattr(lib, fn).restype = C.c_int
+    lib.mps_create_batch.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]
+    lib.mps_get_scene_constants.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+    lib.mps_set_graph.argtypes = [C.c_void_p, C.c_int]
+    lib.mps_set_launch.argtypes = [C.c_void_p, C.c_int, C.c_int]
     lib.mps_sync.argtypes = [C.c_void_p]
     lib.mps_set_stream.argtypes = [C.c_void_p, C.c_void_p]
     lib.mps_timing_enable.argtypes = [C.c_void_p, C.c_int]
@@ -199,17 +204,36 @@ class MPSError(RuntimeError):
 
 
 class Handle:
-    """One simulation behind a C ABI with the handle functions of mps.h."""
+    """One simulation behind a C ABI with the handle functions of mps.h.
 
-    def __init__(self, params: Params, scene: SceneArrays, lib=None, prefix="mps_"):
+    With lists of params and scenes (HIP library only) the handle holds a
+    batch of independent scenes (mps_create_batch); state arrays are then the
+    scenes' arrays concatenated, scene k starting at ``offsets[k]``.
+    """
+
+    def __init__(self, params, scene, lib=None, prefix="mps_"):
         self.lib = lib if lib is not None else load_library()
         self.prefix = prefix
-        self.params = params
-        self.n = scene.n
-        self.dim = params.dim
         self._h = C.c_void_p()
-        s = scene.struct()
-        self._check(self._f("create")(C.byref(params), C.byref(s), C.byref(self._h)))
+        if isinstance(scene, (list, tuple)):
+            if len(scene) != len(params) or not scene:
+                raise ValueError("need one params per scene")
+            self.nscenes = len(scene)
+            self.params = params[0]
+            sizes = [sc.n for sc in scene]
+            parr = (Params * self.nscenes)(*params)
+            self._structs = [sc.struct() for sc in scene]
+            sarr = (Scene * self.nscenes)(*self._structs)
+            self._check(self.lib.mps_create_batch(self.nscenes, parr, sarr, C.byref(self._h)))
+        else:
+            self.nscenes = 1
+            self.params = params
+            sizes = [scene.n]
+            s = scene.struct()
+            self._check(self._f("create")(C.byref(params), C.byref(s), C.byref(self._h)))
+        self.offsets = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+        self.n = int(self.offsets[-1])
+        self.dim = self.params.dim
 
     def _f(self, name):
         return getattr(self.lib, self.prefix + name)
@@ -236,10 +260,29 @@ class Handle:
     def phase(self, ph):
         self._check(self._f("phase")(self._h, ph))
 
-    def constants(self) -> dict:
+    def constants(self, scene=None) -> dict:
         c = Constants()
-        self._check(self._f("get_constants")(self._h, C.byref(c)))
+        if scene is None:
+            self._check(self._f("get_constants")(self._h, C.byref(c)))
+        else:
+            self._check(self.lib.mps_get_scene_constants(self._h, scene, C.byref(c)))
         return c.as_dict()
+
+    def total_cells(self) -> int:
+        if self.nscenes == 1:
+            return self.constants()["ncell"]
+        return sum(self.constants(k)["ncell"] for k in range(self.nscenes))
+
+    def set_graph(self, steps_per_graph: int):
+        """Replay groups of steps from a captured hipGraph (0 = eager)."""
+        self._check(self.lib.mps_set_graph(self._h, steps_per_graph))
+
+    def set_launch(self, block: int, wave_grid: int = 16384):
+        self._check(self.lib.mps_set_launch(self._h, block, wave_grid))
+
+    def set_stream(self, stream: int):
+        """hipStream_t as an integer (e.g. torch.cuda.Stream.cuda_stream); 0 = own."""
+        self._check(self.lib.mps_set_stream(self._h, C.c_void_p(stream or None)))
 
     def state(self) -> dict:
         n = self.n
@@ -258,7 +301,7 @@ class Handle:
 
     def buckets(self, which=BKT_FLUID):
         cnt = C.c_int()
-        ncell = self.constants()["ncell"]
+        ncell = self.total_cells()
         cs = np.zeros(ncell + 1, np.int32)
         self._check(self._f("get_buckets")(self._h, which, cs.ctypes.data, None, C.byref(cnt)))
         ids = np.zeros(max(cnt.value, 1), np.int32)

@@ -111,3 +111,26 @@ def test_device_entry_points_fail_loudly_without_a_gpu():
     sc, p = configs.small2d()
     with pytest.raises(ffi.MPSError, match="no HIP device|HIP"):
         ffi.Handle(p, sc)
+
+
+def test_batch_checks_every_scene_before_touching_the_device():
+    s2, p2 = configs.small2d("fp32")
+    s3, p3 = configs.small3d("fp32")
+    with pytest.raises(ffi.MPSError, match="same dim and precision"):
+        ffi.Handle([p2, p3], [s2, s3])
+    bad = ffi.SceneArrays(typ=s2.typ.copy(), pos=s2.pos, normal=s2.normal,
+                          range_min=s2.range_min, range_max=s2.range_max)
+    bad.typ[0] = 9
+    with pytest.raises(ffi.MPSError, match="scene 1: .*bad type"):
+        ffi.Handle([p2, p2], [s2, bad])
+
+
+def test_cfg3_batch_workload():
+    scenes, params = configs.cfg3()
+    assert len(scenes) == 512
+    nf = np.array([(s.typ == ffi.FLD).sum() for s in scenes])
+    assert 1800 <= nf.min() and nf.max() <= 8400
+    nu = np.array([p.nu for p in params])
+    c = np.array([p.c for p in params])
+    assert 1e-6 <= nu.min() and nu.max() <= 1e-4 and 15 <= c.min() and c.max() <= 30
+    assert all(p.dt == 1e-4 and p.dim == 2 and p.precision == ffi.FP32 for p in params)
