@@ -13,8 +13,12 @@ before the timed region).  Prints one JSON line:
                 flops against the FP32 VALU peak
   cpu_baseline  the FP32 CPU oracle on this host's cores, on a bounded
                 sample of the same workload (first steps of the same scene)
+  batched       512 uneven 2-D dam breaks (1.95k-8.2k fluid each) in one
+                handle, launch shape autotuned, steps replayed from a
+                hipGraph, against the same scenes as single-scene handles
+                stepped one after another with plain launches ("speedup")
 
-Usage: python bench.py [--steps K] [--warmup W] [--gpus 1]
+Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-batch]
 """
 from __future__ import annotations
 
@@ -106,6 +110,58 @@ def cpu_baseline(seconds: float) -> dict:
                       f"{dt:.1f} s, {nf} fluid at start)"}
 
 
+def batched_leg(steps: int, seq_steps: int, log) -> dict:
+    """cfg3: 512 uneven 2-D dam breaks stepped as one batch (autotuned launch
+    shape, hipGraph replay) against the same scenes stepped one after another
+    with plain eager launches of single-scene handles."""
+    from mps_cuda_amd import autotune, configs, ffi
+
+    scenes, params = configs.cfg3("fp32")
+    nf = [int((s.typ == ffi.FLD).sum()) for s in scenes]
+    log(f"[bench] cfg3: {len(scenes)} scenes, {sum(nf)} fluid ({min(nf)}-{max(nf)} per scene)")
+    tuned = autotune.tune(params, scenes, steps=16, use_cache=False, log=log)
+    log(f"[bench] cfg3 autotune: {tuned}")
+    h = ffi.Handle(params, scenes)
+    autotune.apply(h, tuned)
+    h.step(max(4, tuned["graph"]))
+    h.sync()
+    live0 = int((h.state()["typ"] == ffi.FLD).sum())
+    t0 = time.perf_counter()
+    h.step(steps)
+    h.sync()
+    el_b = time.perf_counter() - t0
+    live1 = int((h.state()["typ"] == ffi.FLD).sum())
+    h.close()
+    rate_b = 0.5 * (live0 + live1) * steps / el_b
+    log(f"[bench] cfg3 batched: {steps} steps in {el_b:.3f} s")
+
+    singles = [ffi.Handle(p, s) for p, s in zip(params, scenes)]
+    for g in singles:
+        g.step(2)
+    for g in singles:
+        g.sync()
+    live = 0
+    t0 = time.perf_counter()
+    for g in singles:
+        g.step(seq_steps)
+        g.sync()
+    el_s = time.perf_counter() - t0
+    for g in singles:
+        live += int((g.state()["typ"] == ffi.FLD).sum())
+        g.close()
+    rate_s = live * seq_steps / el_s
+    log(f"[bench] cfg3 sequential: {len(singles)} x {seq_steps} steps in {el_s:.3f} s")
+    return {"workload": "cfg3: 512 independent 2-D dam breaks, FP32, l0 = 0.005, tank 1.0 x 0.6",
+            "scenes": len(scenes), "fluid": int(sum(nf)), "steps": steps,
+            "value": round(rate_b, 1), "unit": "particle-steps/s",
+            "ms_per_step": round(1e3 * el_b / steps, 4),
+            "sequential": {"value": round(rate_s, 1), "steps_per_scene": seq_steps,
+                           "ms_per_scene_step": round(1e3 * el_s / (seq_steps * len(scenes)), 4),
+                           "launch": "plain (eager) launches of single-scene handles, one scene after another"},
+            "speedup": round(rate_b / rate_s, 2),
+            "autotune": {k: tuned[k] for k in ("block", "wave_grid", "graph", "trials")}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -113,6 +169,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-steps", type=int, default=200)
+    ap.add_argument("--seq-steps", type=int, default=10)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -183,6 +242,11 @@ def main():
         print("[bench] CPU baseline ...", file=sys.stderr, flush=True)
         cpu = cpu_baseline(args.cpu_seconds)
 
+    batch = None
+    if not args.no_batch:
+        batch = batched_leg(args.batch_steps, args.seq_steps,
+                            lambda msg: print(msg, file=sys.stderr, flush=True))
+
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "particle-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * elapsed / args.steps, 4),
@@ -193,6 +257,7 @@ def main():
                    "tank": [1.0, 0.6, 0.3], "column": [0.25, 0.5, 0.3]},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "batched": batch,
         "kernels_ms": {k: round(v[0] / v[1], 5) for k, v in sorted(kt.items(), key=lambda kv: -kv[1][0])},
     }
     print(json.dumps(out), flush=True)

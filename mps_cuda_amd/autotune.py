@@ -1,0 +1,116 @@
+"""Launch-shape autotuner for a handle (single scene or batch).
+
+The tuned knobs change only how work is laid onto the GPU, never a result
+(the kernels are bit-identical under every setting):
+
+  block      workgroup size of the per-particle kernels (64, 128, 256).
+             Small scenes waste up to block-1 lanes each in scene-uniform
+             workgroups, large ones prefer fewer, bigger workgroups.
+  wave_grid  workgroups of the one-wave-per-particle mirror kernel
+             (grid-stride over the near-wall particles).
+  graph      steps recorded per hipGraph replay (0 = eager launches).
+
+Search: coordinate descent from the defaults (block, then wave_grid, then
+graph length), each trial timed by the host clock around `steps` steps of a
+probe handle built from the same scenes, so the caller's simulation does not
+advance.  Results are cached in a JSON file keyed by the workload shape.
+"""
+from __future__ import annotations
+
+import json
+import os
+import time
+
+from . import ffi
+
+BLOCKS = (64, 128, 256)
+WAVE_GRIDS = (2048, 8192, 32768)
+GRAPHS = (0, 2, 8, 32)
+DEFAULT = dict(block=256, wave_grid=16384, graph=0)
+
+
+def cache_path() -> str:
+    return os.environ.get("MPS_AUTOTUNE_CACHE",
+                          os.path.join(os.path.expanduser("~"), ".cache", "mps_cuda_amd",
+                                       "autotune.json"))
+
+
+def workload_key(params, scenes) -> str:
+    ps = params if isinstance(params, (list, tuple)) else [params]
+    ss = scenes if isinstance(scenes, (list, tuple)) else [scenes]
+    n = sum(s.n for s in ss)
+    nf = sum(int((s.typ != ffi.WLL).sum()) for s in ss)
+    prec = "fp64" if ps[0].precision == ffi.FP64 else "fp32"
+    return f"d{ps[0].dim}-{prec}-s{len(ss)}-n{n}-f{nf}-l0{ps[0].l0:g}"
+
+
+def _load(path):
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _store(path, table):
+    try:
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        tmp = path + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(table, f, indent=1, sort_keys=True)
+        os.replace(tmp, path)
+    except OSError:
+        pass  # a read-only home only costs a re-tune
+
+
+def time_config(h: ffi.Handle, cfg: dict, steps: int) -> float:
+    """Milliseconds per step of `h` under cfg (warm-up included, untimed)."""
+    h.set_launch(cfg["block"], cfg["wave_grid"])
+    h.set_graph(cfg["graph"])
+    warm = max(2, cfg["graph"])
+    h.step(warm)
+    h.sync()
+    n = max(steps, cfg["graph"]) if cfg["graph"] == 0 else \
+        max(1, steps // cfg["graph"]) * cfg["graph"]
+    t0 = time.perf_counter()
+    h.step(n)
+    h.sync()
+    return (time.perf_counter() - t0) * 1e3 / n
+
+
+def tune(params, scenes, steps: int = 32, use_cache: bool = True, log=None) -> dict:
+    """Best {block, wave_grid, graph, ms_per_step} for this workload."""
+    key = workload_key(params, scenes)
+    path = cache_path()
+    table = _load(path) if use_cache else {}
+    if key in table:
+        return dict(table[key], cached=True)
+    probe = ffi.Handle(params, scenes)
+    trials = []
+    try:
+        best = dict(DEFAULT)
+        best_ms = time_config(probe, best, steps)
+        trials.append((dict(best), best_ms))
+        for knob, values in (("block", BLOCKS), ("wave_grid", WAVE_GRIDS), ("graph", GRAPHS)):
+            for v in values:
+                if v == best[knob]:
+                    continue
+                cfg = dict(best, **{knob: v})
+                ms = time_config(probe, cfg, steps)
+                trials.append((cfg, ms))
+                if log:
+                    log(f"autotune {key} {cfg} {ms:.4f} ms/step")
+                if ms < best_ms:
+                    best, best_ms = cfg, ms
+    finally:
+        probe.close()
+    out = dict(best, ms_per_step=best_ms, trials=len(trials))
+    if use_cache:
+        table[key] = out
+        _store(path, table)
+    return dict(out, cached=False)
+
+
+def apply(h: ffi.Handle, cfg: dict):
+    h.set_launch(cfg["block"], cfg["wave_grid"])
+    h.set_graph(cfg["graph"])

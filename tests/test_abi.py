@@ -134,3 +134,15 @@ def test_cfg3_batch_workload():
     c = np.array([p.c for p in params])
     assert 1e-6 <= nu.min() and nu.max() <= 1e-4 and 15 <= c.min() and c.max() <= 30
     assert all(p.dt == 1e-4 and p.dim == 2 and p.precision == ffi.FP32 for p in params)
+
+
+def test_autotune_cache_round_trip(tmp_path, monkeypatch):
+    from mps_cuda_amd import autotune
+    monkeypatch.setenv("MPS_AUTOTUNE_CACHE", str(tmp_path / "tune.json"))
+    scenes, params = configs.batch_small(3, "fp32")
+    key = autotune.workload_key(params, scenes)
+    assert key.startswith("d2-fp32-s3-")
+    entry = dict(block=128, wave_grid=2048, graph=8, ms_per_step=1.0, trials=9)
+    autotune._store(autotune.cache_path(), {key: entry})
+    got = autotune.tune(params, scenes)  # served from the cache: no device needed
+    assert got["cached"] and got["block"] == 128 and got["graph"] == 8
