@@ -242,6 +242,32 @@ int mps_get_mirrors(mps_handle* h, int* ids, double* pos, double* prs, int* coun
 
 const char* mps_last_error(void);
 
+/* ---- adjoint (no reference counterpart: the reference has no derivative
+ * of the step; these give gradients of a trajectory loss for parameter
+ * identification) ----------------------------------------------------------
+ *
+ * The reverse sweep differentiates the step with every discrete choice of the
+ * forward frozen (neighbour, mirror and contact sets, contact and pressure
+ * clip masks, the P_min argmin, the speed clamp, deletions).  Single-scene
+ * handles only. */
+
+/* Runs nsteps steps (as mps_step) from the current state, keeping the state
+ * every checkpoint_every steps for the reverse sweep. */
+int mps_adjoint_forward(mps_handle* h, int nsteps, int checkpoint_every);
+
+/* Given dL/d(final positions) and dL/d(final velocities) ([3][n], creation
+ * order, either may be NULL = 0), returns dL/d(initial positions) and
+ * dL/d(initial velocities) ([3][n], NULL to skip) and
+ * param_bar[0] = dL/dnu, param_bar[1] = dL/dc (NULL to skip).  Each segment
+ * of checkpoint_every steps is recomputed from its checkpoint with per-step
+ * tapes, then reversed.  The handle's state is the final state again on
+ * return; the sweep may be repeated with other seeds. */
+int mps_adjoint_backward(mps_handle* h, const double* pos_bar, const double* vel_bar, double* pos0_bar,
+                         double* vel0_bar, double* param_bar);
+
+/* Frees checkpoints and tapes. */
+int mps_adjoint_release(mps_handle* h);
+
 /* ---- measurement (no reference counterpart; the reference only times the
  * whole run, main.cu:4-5, 28-36) --------------------------------------- */
 

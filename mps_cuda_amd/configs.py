@@ -10,6 +10,10 @@
         per scene (seed 3): column width U[0.15, 0.35], height U[0.3, 0.6]
         (1.8k-8.4k fluid), nu log-uniform in [1e-6, 1e-4], c U[15, 30];
         dt = 1e-4 (acoustic Courant number c dt / l0 <= 0.6), 1000 steps.
+  cfg4  gradients: cfg0 with initial fluid velocities N(0, 0.01 m/s) per
+        component (seed 4), 400 steps; loss = mean squared distance of the
+        final fluid positions to those of the same run at nu = 1e-4, c = 25.
+        Timing case: cfg1 for 200 steps, a checkpoint every 25 steps.
 Common constants: rho = 1000, nu = 1e-6, g = -9.81, c = 22 m/s, collision
 distance 0.5 l0, restitution 0.2, re = 3.1 l0, rp = 2.1 l0.
 
@@ -110,3 +114,25 @@ def batch_small(n_scenes=5, precision="fp64", seed=11, dim=2):
         scenes.append(sc)
         params.append(p)
     return scenes, params
+
+
+def with_velocity(scene: ffi.SceneArrays, sigma: float, seed: int, dim: int) -> ffi.SceneArrays:
+    """Copy of `scene` with fluid velocities drawn from N(0, sigma) per component."""
+    rng = np.random.default_rng(seed)
+    fl = scene.typ == ffi.FLD
+    vel = np.zeros((3, scene.n))
+    vel[:dim, fl] = rng.normal(0.0, sigma, (dim, int(fl.sum())))
+    return ffi.SceneArrays(typ=scene.typ, pos=scene.pos, normal=scene.normal, vel=vel,
+                           range_min=scene.range_min, range_max=scene.range_max)
+
+
+CFG4_STEPS = 400
+CFG4_TARGET = dict(nu=1e-4, c=25.0)
+
+
+def cfg4(precision="fp64"):
+    """(scene with v0, params at nu = 1e-6, c = 22, params of the target run)."""
+    sc, p = cfg0(precision)
+    sc = with_velocity(sc, 0.01, 4, 2)
+    pt = params_for(sc, 2, 0.01, 5e-4, precision, **CFG4_TARGET)
+    return sc, p, pt

@@ -1,0 +1,593 @@
+// mps_adjoint.h -- reverse-mode (adjoint) kernels of the explicit MPS step.
+//
+// The forward step is the ClcMPS loop body (kernel.cu:1904-1935):
+//   bucket, mirrors A (from x), Vsc a1 (x, v, A), Up1 (vu = v + a1 dt,
+//   x1 = x + vu dt, v1 = clamp(vu)), bucket, Col v2 (x1, v1, walls, A),
+//   Prs P (x1, A), mirrors B (from x1, P), Grad a2 (x1, P, B),
+//   Up2 (vw = v2 + a2 dt, x2 = x1 + a2 dt^2, v3 = clamp(vw)).
+// Its vector-Jacobian product is taken with every discrete choice of the
+// forward frozen: neighbour, mirror and contact sets, the fDT > 0 contact
+// masks, the pressure clip masks, the P_min argmin, the speed-clamp branch
+// and deletions.  Each kernel below is the transpose of one phase.
+//
+// Pair terms are gathered, not scattered: every fluid-fluid relation used by
+// the step (|p| <= re, < rp, d2 < rp2, d2 < rlim2 with fDT > 0) is symmetric,
+// so particle k collects both its own pair term and its neighbours' terms
+// that involve it, with no atomics.  Mirror-parent terms use atomics.
+//
+// Adjoint arrays follow the fluid order of the phase they belong to; the
+// bucket builds are undone by scattering through their permutations.
+#pragma once
+#include "mps_nlist.h"
+
+namespace mps {
+
+template <class real, int DIM>
+__device__ inline void ld3(const Vec<real, DIM>& a, real (&o)[3]) {
+  o[0] = a.x;
+  o[1] = a.y;
+  o[2] = vz(a);
+}
+template <class real, int DIM>
+__device__ inline Vec<real, DIM> st3(const real (&o)[3]) {
+  return mkvec<real, DIM>(o[0], o[1], DIM == 3 ? o[2] : real(0));
+}
+template <class real, int DIM>
+__device__ inline real dotv(const real (&a)[3], const real (&b)[3]) {
+  real s = a[0] * b[0] + a[1] * b[1];
+  if constexpr (DIM == 3) s += a[2] * b[2];
+  return s;
+}
+template <class real, int DIM>
+__device__ inline void atomic_add_vec(Vec<real, DIM>* a, const real (&g)[3]) {
+  atomicAdd(&a->x, g[0]);
+  atomicAdd(&a->y, g[1]);
+  if constexpr (DIM == 3) atomicAdd(&a->z, g[2]);
+}
+
+// Transpose of the speed clamp v = u * (umax / |u|) (applied when |u| > umax):
+// J = (umax / U) (I - u u^T / U^2) is symmetric.
+template <class real, int DIM>
+__device__ inline void clamp_vjp(const real (&u)[3], real U, real umax, real (&g)[3]) {
+  const real ug = dotv<real, DIM>(u, g) / (U * U);
+  const real s = umax / U;
+  for (int k = 0; k < 3; k++) g[k] = s * (g[k] - u[k] * ug);
+}
+
+// Visits the 27 (9 in 2-D) stencil buckets of cell q in array cs.
+template <class real, int DIM, class F>
+__device__ inline void for_stencil(const Cell<real, DIM>& q, const Cst<real>& c, const int* cs, F&& f) {
+  const int zlo = DIM == 3 ? q.iz - 1 : 0, zhi = DIM == 3 ? q.iz + 1 : 0;
+  for (int jz = zlo; jz <= zhi; jz++)
+    for (int jy = q.iy - 1; jy <= q.iy + 1; jy++) {
+      const int b = cs[cell_key(q.ix - 1, jy, jz, c)], e = cs[cell_key(q.ix + 1, jy, jz, c) + 1];
+      for (int j = b; j < e; j++) f(j);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Bookkeeping kernels.
+
+// inv[id] = sorted index of fluid entry with creation index id.
+__global__ void k_inverse(const int* __restrict__ fid, int n, int* __restrict__ inv) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < n) inv[fid[s]] = s;
+}
+
+// Copies the live mirrors (count = *count) into a tape of capacity cap with
+// each mirror's parent as a sorted fluid index; sets *overflow when they do
+// not fit.
+template <class real, int DIM>
+__global__ void k_tape_mirrors(const Vec<real, DIM>* __restrict__ mpos, const int* __restrict__ mid,
+                               const int* __restrict__ count, const int* __restrict__ inv, int cap,
+                               Vec<real, DIM>* __restrict__ tpos, int* __restrict__ tpar,
+                               int* __restrict__ overflow) {
+  const int n = *count;
+  if (n > cap && blockIdx.x == 0 && threadIdx.x == 0) *overflow = 1;
+  const int m_end = min(n, cap);
+  for (int m = blockIdx.x * blockDim.x + threadIdx.x; m < m_end; m += gridDim.x * blockDim.x) {
+    tpos[m] = mpos[m];
+    tpar[m] = inv[mid[m] / kNumMRR];
+  }
+}
+
+// out[perm[s]] = in[s]: transpose of the bucket gather dst[s] = src[perm[s]].
+template <class T>
+__global__ void k_unpermute(const int* __restrict__ perm, int n, const T* __restrict__ in, T* __restrict__ out) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < n) out[perm[s]] = in[s];
+}
+
+// Adjoint seeds from caller arrays [3][n_total] (creation order, double).
+template <class real, int DIM>
+__global__ void k_adj_seed(const int* __restrict__ fid, const int8_t* __restrict__ typ, int nf,
+                           long long n_total, const double* __restrict__ xbar,
+                           const double* __restrict__ vbar, Vec<real, DIM>* __restrict__ xb,
+                           Vec<real, DIM>* __restrict__ vb) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nf) return;
+  const long long id = fid[s];
+  const bool live = typ[s] == kFLD;
+  real a[3], b[3];
+  for (int k = 0; k < 3; k++) {
+    a[k] = live && xbar ? real(xbar[k * n_total + id]) : real(0);
+    b[k] = live && vbar ? real(vbar[k * n_total + id]) : real(0);
+  }
+  xb[s] = st3<real, DIM>(a);
+  vb[s] = st3<real, DIM>(b);
+}
+
+template <class real, int DIM>
+__global__ void k_adj_out(const int* __restrict__ fid, int nf, long long n_total,
+                          const Vec<real, DIM>* __restrict__ xb, const Vec<real, DIM>* __restrict__ vb,
+                          double* __restrict__ gx, double* __restrict__ gv) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nf) return;
+  const long long id = fid[s];
+  real a[3], b[3];
+  ld3(xb[s], a);
+  ld3(vb[s], b);
+  for (int k = 0; k < 3; k++) {
+    if (gx) gx[k * n_total + id] = double(a[k]);
+    if (gv) gv[k * n_total + id] = double(b[k]);
+  }
+}
+
+// *acc += sum of a[0 .. cs[tot]) in a fixed order (one workgroup).
+template <class real>
+__global__ __launch_bounds__(1024) void k_sum_into(const real* __restrict__ a, const int* __restrict__ cs,
+                                                   int tot, double* __restrict__ acc) {
+  __shared__ double sh[1024];
+  const int n = cs[tot];
+  double s = 0;
+  for (int i = threadIdx.x; i < n; i += 1024) s += double(a[i]);
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 512; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *acc += sh[0];
+}
+
+// ---------------------------------------------------------------------------
+// Up2^T (kernel.cu:1815-1829 with d_ChkPcl): in xb = x2bar, vb = v3bar;
+// out ab = a2bar, gb = v2bar; xb becomes the x1bar contribution (x1bar = x2bar).
+template <class real, int DIM>
+__global__ void k_adj_up2(const int* __restrict__ cs, int tot, int nf, const Vec<real, DIM>* __restrict__ v2,
+                          const Vec<real, DIM>* __restrict__ a2, const int8_t* __restrict__ typ_out,
+                          Vec<real, DIM>* __restrict__ xb, const Vec<real, DIM>* __restrict__ vb,
+                          Vec<real, DIM>* __restrict__ ab, Vec<real, DIM>* __restrict__ gb, Cst<real> c) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nf) return;
+  const Vec<real, DIM> zero = mkvec<real, DIM>(0, 0, 0);
+  if (i >= cs[tot] || typ_out[i] != kFLD) {  // deleted by this step: outputs are constants
+    xb[i] = zero;
+    ab[i] = zero;
+    gb[i] = zero;
+    return;
+  }
+  real v[3], a[3], g[3], x[3], u[3];
+  ld3(v2[i], v);
+  ld3(a2[i], a);
+  ld3(vb[i], g);
+  ld3(xb[i], x);
+  for (int k = 0; k < 3; k++) u[k] = v[k] + a[k] * c.dt;
+  const real U = sqrt(norm2<real, DIM>(u[0], u[1], u[2]));
+  if (U > c.umax) clamp_vjp<real, DIM>(u, U, c.umax, g);
+  real ao[3];
+  for (int k = 0; k < 3; k++) ao[k] = g[k] * c.dt + x[k] * c.dt * c.dt;
+  ab[i] = st3<real, DIM>(ao);
+  gb[i] = st3<real, DIM>(g);
+}
+
+// Grad^T, first pass (kernel.cu:1651-1794).  Per particle i: the P_min
+// argmin over {i} and its fluid neighbours with d2 < rp2 (first strictly
+// smaller in visiting order, as the forward), s_i = K sum f(d) (p . abar_i)
+// over all its terms, and the mirror terms: x1bar_i -= G, zbar_m += G,
+// Pbar_m += K f (p . abar_i), with K = -(1/rho)(dim/n0_grad),
+// f(d) = (rp/d - d/rp)/d^2, G = K (P_m - P_min)(f abar + f'(d)/d (p.abar) p).
+template <class real, int DIM>
+__global__ __launch_bounds__(kBlock) void k_adj_grad_pre(
+    const Vec<real, DIM>* __restrict__ pos, const real* __restrict__ prs, const int* __restrict__ fcs,
+    int nf, const int* __restrict__ list, const int* __restrict__ count,
+    const Vec<real, DIM>* __restrict__ mpos, const int* __restrict__ mcs, const uint8_t* __restrict__ mnear,
+    const int* __restrict__ mpar, const Vec<real, DIM>* __restrict__ ab, Cst<real> c,
+    int* __restrict__ argmin, real* __restrict__ sg, real* __restrict__ pmin_out,
+    Vec<real, DIM>* __restrict__ xb, Vec<real, DIM>* __restrict__ zb, real* __restrict__ pbm) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= fcs[c.tot]) return;
+  real x[3], a[3];
+  ld3(pos[i], x);
+  ld3(ab[i], a);
+  const real rp = c.rp, rp2 = c.rp2, K = c.invro * c.grad_a;
+  real pmin = prs[i];
+  int am = i;
+  const Cell<real, DIM> q = cell_of<real, DIM>(pos[i], c);
+  for_fluid_near<real, DIM>(i, nf, list, count, fcs, q, c, [&](int j) {
+    real y[3];
+    ld3(pos[j], y);
+    if (norm2<real, DIM>(y[0] - x[0], y[1] - x[1], y[2] - x[2]) < rp2 && prs[j] < pmin) {
+      pmin = prs[j];
+      am = j;
+    }
+  });
+  real s = 0;
+  for_fluid_near<real, DIM>(i, nf, list, count, fcs, q, c, [&](int j) {
+    real y[3], p[3];
+    ld3(pos[j], y);
+    for (int k = 0; k < 3; k++) p[k] = y[k] - x[k];
+    const real d2 = norm2<real, DIM>(p[0], p[1], p[2]);
+    if (!(d2 < rp2)) return;
+    const real d = sqrt(d2);
+    s += ((rp / d - d / rp) / d2) * dotv<real, DIM>(p, a);
+  });
+  real xacc[3] = {0, 0, 0};
+  if (mnear[cell_key(q.ix, q.iy, q.iz, c)])
+    for_stencil<real, DIM>(q, c, mcs, [&](int m) {
+      real y[3], p[3];
+      ld3(mpos[m], y);
+      for (int k = 0; k < 3; k++) p[k] = y[k] - x[k];
+      const real d2 = norm2<real, DIM>(p[0], p[1], p[2]);
+      if (!(d2 < rp2)) return;
+      const real d = sqrt(d2);
+      const real f = (rp / d - d / rp) / d2;
+      const real fp = real(-3) * rp / (d2 * d2) + real(1) / (rp * d2);
+      const real pa = dotv<real, DIM>(p, a);
+      const real t = K * (prs[mpar[m]] - pmin);
+      real G[3];
+      for (int k = 0; k < 3; k++) G[k] = t * (f * a[k] + fp / d * pa * p[k]);
+      for (int k = 0; k < 3; k++) xacc[k] -= G[k];
+      atomic_add_vec<real, DIM>(&zb[m], G);
+      atomicAdd(&pbm[m], K * f * pa);
+      s += f * pa;
+    });
+  argmin[i] = am;
+  sg[i] = K * s;
+  pmin_out[i] = pmin;
+  real xo[3];
+  ld3(xb[i], xo);
+  for (int k = 0; k < 3; k++) xo[k] += xacc[k];
+  xb[i] = st3<real, DIM>(xo);
+}
+
+// Grad^T, gather pass: fluid pair terms of k's own acceleration and of its
+// neighbours' accelerations that involve k, and Pbar_k.
+template <class real, int DIM>
+__global__ __launch_bounds__(kBlock) void k_adj_grad(
+    const Vec<real, DIM>* __restrict__ pos, const real* __restrict__ prs, const int* __restrict__ fcs,
+    int nf, const int* __restrict__ list, const int* __restrict__ count,
+    const Vec<real, DIM>* __restrict__ ab, const int* __restrict__ argmin, const real* __restrict__ sg,
+    const real* __restrict__ pmin, Cst<real> c, Vec<real, DIM>* __restrict__ xb, real* __restrict__ pb) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= fcs[c.tot]) return;
+  real x[3], a[3];
+  ld3(pos[k], x);
+  ld3(ab[k], a);
+  const real rp = c.rp, rp2 = c.rp2, K = c.invro * c.grad_a;
+  const real Pk = prs[k], Pmk = pmin[k];
+  real xacc[3] = {0, 0, 0}, pacc = 0;
+  const Cell<real, DIM> q = cell_of<real, DIM>(pos[k], c);
+  for_fluid_near<real, DIM>(k, nf, list, count, fcs, q, c, [&](int j) {
+    real y[3], p[3], aj[3];
+    ld3(pos[j], y);
+    for (int t = 0; t < 3; t++) p[t] = y[t] - x[t];
+    const real d2 = norm2<real, DIM>(p[0], p[1], p[2]);
+    if (!(d2 < rp2)) return;
+    ld3(ab[j], aj);
+    const real d = sqrt(d2);
+    const real f = (rp / d - d / rp) / d2;
+    const real fp = real(-3) * rp / (d2 * d2) + real(1) / (rp * d2);
+    const real pak = dotv<real, DIM>(p, a), paj = dotv<real, DIM>(p, aj);
+    const real t1 = K * (prs[j] - Pmk), t2 = K * (Pk - pmin[j]);
+    for (int t = 0; t < 3; t++)
+      xacc[t] += -t1 * (f * a[t] + fp / d * pak * p[t]) + t2 * (f * aj[t] + fp / d * paj * p[t]);
+    pacc -= K * f * paj;
+    if (argmin[j] == k) pacc -= sg[j];
+  });
+  if (argmin[k] == k) pacc -= sg[k];
+  real xo[3];
+  ld3(xb[k], xo);
+  for (int t = 0; t < 3; t++) xo[t] += xacc[t];
+  xb[k] = st3<real, DIM>(xo);
+  pb[k] = pacc;
+}
+
+// Mirror^T (kernel.cu:811-888): a mirror is the reflection of its parent x
+// across a wall plane, y = x - 2 ((x - w) . n / |n|) n, so dy/dx = I - 2 n n^T
+// for the generated (unit) normals; n is the direction of x - y.  Adds
+// J^T ybar to the parent's xbar and (B mirrors) the mirror pressure adjoint
+// to the parent's Pbar.
+template <class real, int DIM>
+__global__ void k_adj_mirror_parent(const Vec<real, DIM>* __restrict__ mpos, const int* __restrict__ count,
+                                    const int* __restrict__ mpar, const Vec<real, DIM>* __restrict__ xpar,
+                                    const Vec<real, DIM>* __restrict__ yb, const real* __restrict__ pbm,
+                                    Vec<real, DIM>* __restrict__ xb, real* __restrict__ pb) {
+  const int n = *count;
+  for (int m = blockIdx.x * blockDim.x + threadIdx.x; m < n; m += gridDim.x * blockDim.x) {
+    const int qi = mpar[m];
+    real y[3], x[3], g[3], nv[3];
+    ld3(mpos[m], y);
+    ld3(xpar[qi], x);
+    ld3(yb[m], g);
+    for (int k = 0; k < 3; k++) nv[k] = x[k] - y[k];
+    const real nn = sqrt(norm2<real, DIM>(nv[0], nv[1], nv[2]));
+    if (nn > real(0)) {
+      for (int k = 0; k < 3; k++) nv[k] /= nn;
+      const real ng = dotv<real, DIM>(nv, g);
+      for (int k = 0; k < 3; k++) g[k] -= real(2) * ng * nv[k];
+    }
+    atomic_add_vec<real, DIM>(&xb[qi], g);
+    if (pbm) atomicAdd(&pb[qi], pbm[m]);
+  }
+}
+
+// Prs^T pointwise part (kernel.cu:1392-1410): nbar = Pbar dP/dn where the
+// clip is inactive, and the sound-speed term Pbar dP/dc = Pbar 2 P / c.
+template <class real>
+__global__ void k_adj_prs_pre(const int* __restrict__ cs, int tot, int nf, const real* __restrict__ nd,
+                              const real* __restrict__ prs, const real* __restrict__ pb, Cst<real> c,
+                              real snd, real* __restrict__ nbar, real* __restrict__ cpart) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nf) return;
+  if (i >= cs[tot]) {
+    nbar[i] = real(0);
+    cpart[i] = real(0);
+    return;
+  }
+  const real n = nd[i];
+  const real pressure = real(n > c.n0_grad) * (n - c.n0_grad) * c.prs_coef * c.mass;
+  const bool active = n > c.n0_grad && pressure <= c.pmax && pressure >= real(0);
+  nbar[i] = active ? pb[i] * c.prs_coef * c.mass : real(0);
+  cpart[i] = active ? pb[i] * real(2) * prs[i] / snd : real(0);
+}
+
+// Prs^T gather (kernel.cu:1306-1391): n_i = sum w(d) over fluid and A
+// mirrors with sqrt(d2) < rp, w(d) = rp/d - d/rp, w'(d) = -rp/d^2 - 1/rp.
+template <class real, int DIM>
+__global__ __launch_bounds__(kBlock) void k_adj_prs(
+    const Vec<real, DIM>* __restrict__ pos, const int* __restrict__ fcs, int nf,
+    const int* __restrict__ list, const int* __restrict__ count, const real* __restrict__ nbar,
+    const Vec<real, DIM>* __restrict__ mpos, const int* __restrict__ mcs, const uint8_t* __restrict__ mnear,
+    Cst<real> c, Vec<real, DIM>* __restrict__ xb, Vec<real, DIM>* __restrict__ yb) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= fcs[c.tot]) return;
+  real x[3];
+  ld3(pos[k], x);
+  const real rp = c.rp, nk = nbar[k];
+  real xacc[3] = {0, 0, 0};
+  const Cell<real, DIM> q = cell_of<real, DIM>(pos[k], c);
+  for_fluid_near<real, DIM>(k, nf, list, count, fcs, q, c, [&](int j) {
+    real y[3], p[3];
+    ld3(pos[j], y);
+    for (int t = 0; t < 3; t++) p[t] = y[t] - x[t];
+    const real d2 = norm2<real, DIM>(p[0], p[1], p[2]);
+    if (!(d2 <= c.rp2_hi)) return;
+    const real d = sqrt(d2);
+    if (!(d < rp)) return;
+    const real co = (nk + nbar[j]) * (-rp / d2 - real(1) / rp) / d;
+    for (int t = 0; t < 3; t++) xacc[t] -= co * p[t];
+  });
+  if (mnear[cell_key(q.ix, q.iy, q.iz, c)])
+    for_stencil<real, DIM>(q, c, mcs, [&](int m) {
+      real y[3], p[3];
+      ld3(mpos[m], y);
+      for (int t = 0; t < 3; t++) p[t] = y[t] - x[t];
+      const real d2 = norm2<real, DIM>(p[0], p[1], p[2]);
+      const real d = sqrt(d2);
+      if (!(d < rp)) return;
+      const real co = nk * (-rp / d2 - real(1) / rp) / d;
+      real g[3];
+      for (int t = 0; t < 3; t++) {
+        g[t] = co * p[t];
+        xacc[t] -= g[t];
+      }
+      atomic_add_vec<real, DIM>(&yb[m], g);
+    });
+  real xo[3];
+  ld3(xb[k], xo);
+  for (int t = 0; t < 3; t++) xo[t] += xacc[t];
+  xb[k] = st3<real, DIM>(xo);
+}
+
+// Col^T (kernel.cu:1168-1287): v2_i = v1_i - sum p kap (p . u) / |p|^2 over
+// active contacts, p = x_j - x_i, u = v_i - v_j.  With gbar the v2 adjoint:
+// v1bar_k = gbar_k - sum kap p (p . (gbar_k - gbar_j)) / |p|^2 and
+// x1bar_k += F(gbar_k - gbar_j), F(g) = kap/|p|^2 [g (p.u) + u (p.g)]
+// - 2 kap (p.u)(p.g)/|p|^4 p; walls and mirrors contribute with gbar_j = 0.
+template <class real, int DIM>
+__global__ __launch_bounds__(kBlock) void k_adj_col(
+    const Vec<real, DIM>* __restrict__ pos, const Vec<real, DIM>* __restrict__ vel,
+    const int* __restrict__ fcs, int nf, const int* __restrict__ list, const int* __restrict__ count,
+    const Vec<real, DIM>* __restrict__ wpos, const Vec<real, DIM>* __restrict__ wvel,
+    const int* __restrict__ wcs, const Vec<real, DIM>* __restrict__ mpos, const int* __restrict__ mcs,
+    const uint8_t* __restrict__ wnear, const uint8_t* __restrict__ mnear,
+    const Vec<real, DIM>* __restrict__ gb, Cst<real> c, Vec<real, DIM>* __restrict__ xb,
+    Vec<real, DIM>* __restrict__ v1b, Vec<real, DIM>* __restrict__ yb) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= fcs[c.tot]) return;
+  real x[3], v[3], g[3];
+  ld3(pos[k], x);
+  ld3(vel[k], v);
+  ld3(gb[k], g);
+  const real kap = c.col * c.mass / (c.mass + c.mass);
+  real vb[3] = {g[0], g[1], g[2]}, xacc[3] = {0, 0, 0};
+  // one contact; returns false when inactive, else F(gd) in F
+  auto contact = [&](const real (&p)[3], const real (&u)[3], const real (&gd)[3], real lim2,
+                     real (&F)[3]) -> bool {
+    const real d2 = norm2<real, DIM>(p[0], p[1], p[2]);
+    if (!(d2 < lim2)) return false;
+    const real pu = dotv<real, DIM>(u, p);
+    if (!(pu > real(0))) return false;
+    const real kp = kap / d2, pg = dotv<real, DIM>(p, gd);
+    for (int t = 0; t < 3; t++) {
+      vb[t] -= kp * pg * p[t];
+      F[t] = kp * (gd[t] * pu + u[t] * pg) - real(2) * kp * pu * pg / d2 * p[t];
+      xacc[t] += F[t];
+    }
+    return true;
+  };
+  const Cell<real, DIM> q = cell_of<real, DIM>(pos[k], c);
+  for_fluid_near<real, DIM>(k, nf, list, count, fcs, q, c, [&](int j) {
+    real y[3], vj[3], gj[3], p[3], u[3], gd[3], F[3];
+    ld3(pos[j], y);
+    for (int t = 0; t < 3; t++) p[t] = y[t] - x[t];
+    if (!(norm2<real, DIM>(p[0], p[1], p[2]) < c.rlim2)) return;
+    ld3(vel[j], vj);
+    ld3(gb[j], gj);
+    for (int t = 0; t < 3; t++) {
+      u[t] = v[t] - vj[t];
+      gd[t] = g[t] - gj[t];
+    }
+    contact(p, u, gd, c.rlim2, F);
+  });
+  const int hk = cell_key(q.ix, q.iy, q.iz, c);
+  if (wnear[hk])
+    for_stencil<real, DIM>(q, c, wcs, [&](int j) {
+      real y[3], vj[3], p[3], u[3], F[3];
+      ld3(wpos[j], y);
+      ld3(wvel[j], vj);
+      for (int t = 0; t < 3; t++) {
+        p[t] = y[t] - x[t];
+        u[t] = v[t] - vj[t];
+      }
+      contact(p, u, g, c.rlim_wall2, F);
+    });
+  if (mnear[hk])
+    for_stencil<real, DIM>(q, c, mcs, [&](int m) {
+      real y[3], p[3], F[3];
+      ld3(mpos[m], y);
+      for (int t = 0; t < 3; t++) p[t] = y[t] - x[t];
+      if (contact(p, v, g, c.rlim2, F)) {
+        for (int t = 0; t < 3; t++) F[t] = -F[t];
+        atomic_add_vec<real, DIM>(&yb[m], F);
+      }
+    });
+  real xo[3];
+  ld3(xb[k], xo);
+  for (int t = 0; t < 3; t++) xo[t] += xacc[t];
+  xb[k] = st3<real, DIM>(xo);
+  v1b[k] = st3<real, DIM>(vb);
+}
+
+// Up1^T (kernel.cu:1138-1149 with d_ChkPcl): in x1bar, v1bar (order of the
+// first bucket build); out xbar = x1bar, vbar = vubar, abar = vubar dt with
+// vubar = clamp^T v1bar + x1bar dt (x1 = x + vu dt uses the unclamped vu).
+template <class real, int DIM>
+__global__ void k_adj_up1(const int* __restrict__ cs, int tot, int nf, const Vec<real, DIM>* __restrict__ v,
+                          const Vec<real, DIM>* __restrict__ a1, const int8_t* __restrict__ typ_out,
+                          const Vec<real, DIM>* __restrict__ x1b, const Vec<real, DIM>* __restrict__ v1b,
+                          Cst<real> c, Vec<real, DIM>* __restrict__ xb, Vec<real, DIM>* __restrict__ vb,
+                          Vec<real, DIM>* __restrict__ ab) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nf) return;
+  const Vec<real, DIM> zero = mkvec<real, DIM>(0, 0, 0);
+  if (i >= cs[tot] || typ_out[i] != kFLD) {
+    xb[i] = zero;
+    vb[i] = zero;
+    ab[i] = zero;
+    return;
+  }
+  real vv[3], a[3], g[3], x[3], u[3];
+  ld3(v[i], vv);
+  ld3(a1[i], a);
+  ld3(v1b[i], g);
+  ld3(x1b[i], x);
+  for (int k = 0; k < 3; k++) u[k] = vv[k] + a[k] * c.dt;
+  const real U = sqrt(norm2<real, DIM>(u[0], u[1], u[2]));
+  if (U > c.umax) clamp_vjp<real, DIM>(u, U, c.umax, g);
+  real vu[3], ao[3];
+  for (int k = 0; k < 3; k++) {
+    vu[k] = g[k] + x[k] * c.dt;
+    ao[k] = vu[k] * c.dt;
+  }
+  xb[i] = x1b[i];
+  vb[i] = st3<real, DIM>(vu);
+  ab[i] = st3<real, DIM>(ao);
+}
+
+// Vsc^T (kernel.cu:1013-1119): a_i = kv [sum_j h(r)(v_j - v_i) - sum_m h(r) v_i] + g
+// with h(r) = 2 l0 / r^3 over fluid j and A mirrors with r <= re, kv = nu/n0.
+// Gathers vbar_k += kv sum h (abar_j - abar_k) - kv sum_m h abar_k,
+// xbar_k -= kv h'(r)/r ((abar_k - abar_j) . (v_j - v_k)) p, the mirror
+// position terms, and nupart_k = abar_k . S_k / n0 (S_k the bracket).
+template <class real, int DIM>
+__global__ __launch_bounds__(kBlock) void k_adj_vsc(
+    const Vec<real, DIM>* __restrict__ pos, const Vec<real, DIM>* __restrict__ vel,
+    const int* __restrict__ fcs, const Vec<real, DIM>* __restrict__ mpos, const int* __restrict__ mcs,
+    const uint8_t* __restrict__ mnear, const Vec<real, DIM>* __restrict__ ab, Cst<real> c, real inv_n0,
+    Vec<real, DIM>* __restrict__ xb, Vec<real, DIM>* __restrict__ vb, Vec<real, DIM>* __restrict__ yb,
+    real* __restrict__ nupart) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= fcs[c.tot]) return;
+  real x[3], v[3], a[3];
+  ld3(pos[k], x);
+  ld3(vel[k], v);
+  ld3(ab[k], a);
+  const real kn = c.n0knm, l0 = c.l0, r = c.r;
+  real vacc[3] = {0, 0, 0}, xacc[3] = {0, 0, 0}, S[3] = {0, 0, 0};
+  const Cell<real, DIM> q = cell_of<real, DIM>(pos[k], c);
+  const int zlo = DIM == 3 ? q.iz - 1 : 0, zhi = DIM == 3 ? q.iz + 1 : 0;
+  for (int jz = zlo; jz <= zhi; jz++)
+    for (int jy = q.iy - 1; jy <= q.iy + 1; jy++) {
+      const int b = fcs[cell_key(q.ix - 1, jy, jz, c)], e = fcs[cell_key(q.ix + 1, jy, jz, c) + 1];
+      for_hits(
+          b, e,
+          [&](int j) {
+            const Vec<real, DIM> pj = pos[j];
+            return j != k && norm2<real, DIM>(pj.x - x[0], pj.y - x[1], vz(pj) - x[2]) <= c.r2_hi;
+          },
+          [&](int j) {
+            real y[3], p[3], vj[3], aj[3], dv[3], da[3];
+            ld3(pos[j], y);
+            for (int t = 0; t < 3; t++) p[t] = y[t] - x[t];
+            const real d2 = norm2<real, DIM>(p[0], p[1], p[2]);
+            const real d = sqrt(d2);
+            if (!(d <= r)) return;
+            ld3(vel[j], vj);
+            ld3(ab[j], aj);
+            const real h = real(2) * l0 / (d2 * d);
+            const real hp = real(-6) * l0 / (d2 * d2);
+            for (int t = 0; t < 3; t++) {
+              dv[t] = vj[t] - v[t];
+              da[t] = a[t] - aj[t];
+              vacc[t] -= kn * h * da[t];
+              S[t] += h * dv[t];
+            }
+            const real w = kn * hp / d * dotv<real, DIM>(da, dv);
+            for (int t = 0; t < 3; t++) xacc[t] -= w * p[t];
+          });
+    }
+  if (mnear[cell_key(q.ix, q.iy, q.iz, c)])
+    for_stencil<real, DIM>(q, c, mcs, [&](int m) {
+      real y[3], p[3];
+      ld3(mpos[m], y);
+      for (int t = 0; t < 3; t++) p[t] = y[t] - x[t];
+      const real d2 = norm2<real, DIM>(p[0], p[1], p[2]);
+      const real d = sqrt(d2);
+      if (!(d <= r)) return;
+      const real h = real(2) * l0 / (d2 * d);
+      const real hp = real(-6) * l0 / (d2 * d2);
+      const real sm = -kn * hp / d * dotv<real, DIM>(a, v);
+      real g[3];
+      for (int t = 0; t < 3; t++) {
+        vacc[t] -= kn * h * a[t];
+        S[t] -= h * v[t];
+        g[t] = sm * p[t];
+        xacc[t] -= g[t];
+      }
+      atomic_add_vec<real, DIM>(&yb[m], g);
+    });
+  real xo[3], vo[3];
+  ld3(xb[k], xo);
+  ld3(vb[k], vo);
+  for (int t = 0; t < 3; t++) {
+    xo[t] += xacc[t];
+    vo[t] += vacc[t];
+  }
+  xb[k] = st3<real, DIM>(xo);
+  vb[k] = st3<real, DIM>(vo);
+  nupart[k] = inv_n0 * dotv<real, DIM>(a, S);
+}
+
+}  // namespace mps

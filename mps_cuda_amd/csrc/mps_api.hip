@@ -19,6 +19,7 @@
 #include "../../include/mps.h"
 #include "mps_host.h"
 #include "mps_kernels.h"
+#include "mps_adjoint.h"
 #include "mps_nlist.h"
 
 namespace {
@@ -58,6 +59,10 @@ struct ImplBase {
   virtual int set_launch(int block, int wave_grid) = 0;
   virtual int set_graph(int steps) = 0;
   virtual int set_stream(hipStream_t s) = 0;
+  virtual int adj_forward(int nsteps, int every) = 0;
+  virtual int adj_backward(const double* xbar, const double* vbar, double* gx0, double* gv0,
+                           double* gp) = 0;
+  virtual void adj_release() = 0;
   hipStream_t stream = nullptr;
   hipStream_t own_stream = nullptr;
   std::vector<void*> allocs;
@@ -211,6 +216,7 @@ struct Impl : ImplBase {
 
   ~Impl() override {
     drop_graph();
+    adj_release();
     free_all();
   }
 
@@ -902,6 +908,323 @@ struct Impl : ImplBase {
     return MPS_OK;
   }
 
+
+  // ---- adjoint: checkpoints, per-step tapes, reverse sweep -------------------
+  struct MirrorTape {
+    V* pos = nullptr;
+    int *par = nullptr, *cs = nullptr;
+    uint8_t* near = nullptr;
+  };
+  struct TapeSlot {
+    int *perm1, *fcs1, *perm2, *fcs2, *nlist, *ncount;
+    V *x, *v, *a1, *x1, *v1, *v2, *a2;
+    real *P, *nd;
+    int8_t *typ1, *typ2;
+    MirrorTape A, B;
+  };
+  struct Ckpt {
+    V *pos, *vel;
+    real *prs, *pav, *nd, *wpav;
+    int8_t* typ;
+    int *id, *sid;
+  };
+  std::vector<void*> adj_allocs, adj_mallocs;  // fixed-size / mirror-sized adjoint memory
+  std::vector<Ckpt> ckpts;
+  Ckpt ck_end{};
+  std::vector<TapeSlot> tape;
+  int adj_steps = 0, adj_every = 0, tape_mcap = 0;
+  bool adj_ready = false;
+  V *xbA = nullptr, *xbB = nullptr, *vbA = nullptr, *vbB = nullptr, *abuf = nullptr, *gbuf = nullptr;
+  V *yAb = nullptr, *zBb = nullptr;
+  real *Pb = nullptr, *nbar = nullptr, *sg = nullptr, *pminb = nullptr, *PBb = nullptr;
+  real *nupart = nullptr, *cpart = nullptr;
+  int *argm = nullptr, *inv = nullptr, *oflow = nullptr;
+  double *acc2 = nullptr, *seed_x = nullptr, *seed_v = nullptr, *gout_x = nullptr, *gout_v = nullptr;
+
+  template <class T>
+  int aalloc(std::vector<void*>& list, T** p, size_t n) {
+    hipError_t e = hipMalloc((void**)p, std::max<size_t>(n * sizeof(T), 256));
+    if (e != hipSuccess)
+      return set_err(e == hipErrorOutOfMemory ? MPS_E_OOM : MPS_E_HIP,
+                     std::string("hipMalloc (adjoint): ") + hipGetErrorString(e));
+    list.push_back(*p);
+    return MPS_OK;
+  }
+  static void free_list(std::vector<void*>& l) {
+    for (void* p : l) (void)hipFree(p);
+    l.clear();
+  }
+  void adj_release() override {
+    if (!adj_allocs.empty() || !adj_mallocs.empty()) (void)hipStreamSynchronize(stream);
+    free_list(adj_mallocs);
+    free_list(adj_allocs);
+    ckpts.clear();
+    tape.clear();
+    ck_end = Ckpt{};
+    adj_ready = false;
+    tape_mcap = 0;
+  }
+
+  template <class T>
+  void d2d(T* dst, const T* src, size_t n) {
+    if (n) (void)hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToDevice, stream);
+  }
+
+  int alloc_ckpt(Ckpt& k) {
+    int rc;
+    if ((rc = aalloc(adj_allocs, &k.pos, nf)) || (rc = aalloc(adj_allocs, &k.vel, nf)) ||
+        (rc = aalloc(adj_allocs, &k.prs, nf)) || (rc = aalloc(adj_allocs, &k.pav, nf)) ||
+        (rc = aalloc(adj_allocs, &k.nd, nf)) || (rc = aalloc(adj_allocs, &k.wpav, nw)) ||
+        (rc = aalloc(adj_allocs, &k.typ, nf)) || (rc = aalloc(adj_allocs, &k.id, nf)) ||
+        (rc = aalloc(adj_allocs, &k.sid, nf)))
+      return rc;
+    return MPS_OK;
+  }
+  void save(Ckpt& k) {
+    d2d(k.pos, fpos[0], nf);
+    d2d(k.vel, fvel[0], nf);
+    d2d(k.prs, fprs[0], nf);
+    d2d(k.pav, fpav[0], nf);
+    d2d(k.nd, fnd[0], nf);
+    d2d(k.wpav, wpav, nw);
+    d2d(k.typ, ftyp[0], nf);
+    d2d(k.id, fid[0], nf);
+    d2d(k.sid, fsid[0], nf);
+  }
+  void restore(const Ckpt& k) {
+    d2d(fpos[0], k.pos, nf);
+    d2d(fvel[0], k.vel, nf);
+    d2d(fprs[0], k.prs, nf);
+    d2d(fpav[0], k.pav, nf);
+    d2d(fnd[0], k.nd, nf);
+    d2d(wpav, k.wpav, nw);
+    d2d(ftyp[0], k.typ, nf);
+    d2d(fid[0], k.id, nf);
+    d2d(fsid[0], k.sid, nf);
+    acc_live = false;
+  }
+
+  // Runs nsteps fused steps from the current state, keeping the state every
+  // `every` steps (and the final one) for the reverse sweep.
+  int adj_forward(int nsteps, int every) override {
+    if (S != 1) return set_err(MPS_E_STATE, "the adjoint runs on single-scene handles");
+    if (nsteps < 1 || every < 1) return set_err(MPS_E_ARG, "need nsteps >= 1 and checkpoint_every >= 1");
+    if (nf == 0) return set_err(MPS_E_STATE, "no fluid particles");
+    drop_graph();
+    adj_release();
+    step_eager(0);  // clears a phase-mode acceleration
+    const int nck = (nsteps + every - 1) / every;
+    ckpts.resize(nck);
+    int rc;
+    for (auto& k : ckpts)
+      if ((rc = alloc_ckpt(k))) return rc;
+    if ((rc = alloc_ckpt(ck_end))) return rc;
+    for (int k = 0; k < nck; k++) {
+      save(ckpts[k]);
+      step_eager(std::min(every, nsteps - k * every));
+    }
+    save(ck_end);
+    adj_steps = nsteps;
+    adj_every = every;
+    adj_ready = true;
+    return check_launch();
+  }
+
+  int alloc_tape_mirrors() {
+    free_list(adj_mallocs);
+    int rc;
+    for (auto& T : tape)
+      for (MirrorTape* M : {&T.A, &T.B})
+        if ((rc = aalloc(adj_mallocs, &M->pos, tape_mcap)) || (rc = aalloc(adj_mallocs, &M->par, tape_mcap)))
+          return rc;
+    if ((rc = aalloc(adj_mallocs, &yAb, tape_mcap)) || (rc = aalloc(adj_mallocs, &zBb, tape_mcap)) ||
+        (rc = aalloc(adj_mallocs, &PBb, tape_mcap)))
+      return rc;
+    return MPS_OK;
+  }
+
+  int alloc_adjoint_work() {
+    int rc;
+    tape.assign(adj_every, TapeSlot{});
+    for (auto& T : tape) {
+      if ((rc = aalloc(adj_allocs, &T.perm1, nf)) || (rc = aalloc(adj_allocs, &T.fcs1, tot + 2)) ||
+          (rc = aalloc(adj_allocs, &T.perm2, nf)) || (rc = aalloc(adj_allocs, &T.fcs2, tot + 2)) ||
+          (rc = aalloc(adj_allocs, &T.nlist, (size_t)mps::kNList * nf)) ||
+          (rc = aalloc(adj_allocs, &T.ncount, nf)) || (rc = aalloc(adj_allocs, &T.x, nf)) ||
+          (rc = aalloc(adj_allocs, &T.v, nf)) || (rc = aalloc(adj_allocs, &T.a1, nf)) ||
+          (rc = aalloc(adj_allocs, &T.x1, nf)) || (rc = aalloc(adj_allocs, &T.v1, nf)) ||
+          (rc = aalloc(adj_allocs, &T.v2, nf)) || (rc = aalloc(adj_allocs, &T.a2, nf)) ||
+          (rc = aalloc(adj_allocs, &T.P, nf)) || (rc = aalloc(adj_allocs, &T.nd, nf)) ||
+          (rc = aalloc(adj_allocs, &T.typ1, nf)) || (rc = aalloc(adj_allocs, &T.typ2, nf)))
+        return rc;
+      for (MirrorTape* M : {&T.A, &T.B})
+        if ((rc = aalloc(adj_allocs, &M->cs, tot + 2)) || (rc = aalloc(adj_allocs, &M->near, tot)))
+          return rc;
+    }
+    const size_t n3 = 3 * (size_t)n_total;
+    if ((rc = aalloc(adj_allocs, &xbA, nf)) || (rc = aalloc(adj_allocs, &xbB, nf)) ||
+        (rc = aalloc(adj_allocs, &vbA, nf)) || (rc = aalloc(adj_allocs, &vbB, nf)) ||
+        (rc = aalloc(adj_allocs, &abuf, nf)) || (rc = aalloc(adj_allocs, &gbuf, nf)) ||
+        (rc = aalloc(adj_allocs, &Pb, nf)) || (rc = aalloc(adj_allocs, &nbar, nf)) ||
+        (rc = aalloc(adj_allocs, &sg, nf)) || (rc = aalloc(adj_allocs, &pminb, nf)) ||
+        (rc = aalloc(adj_allocs, &nupart, nf)) || (rc = aalloc(adj_allocs, &cpart, nf)) ||
+        (rc = aalloc(adj_allocs, &argm, nf)) || (rc = aalloc(adj_allocs, &inv, n_total)) ||
+        (rc = aalloc(adj_allocs, &oflow, 1)) || (rc = aalloc(adj_allocs, &acc2, 2)) ||
+        (rc = aalloc(adj_allocs, &seed_x, n3)) || (rc = aalloc(adj_allocs, &seed_v, n3)) ||
+        (rc = aalloc(adj_allocs, &gout_x, n3)) || (rc = aalloc(adj_allocs, &gout_v, n3)))
+      return rc;
+    int m = 0;
+    HIP_TRY(hipMemcpyAsync(&m, mcs + tot, sizeof(int), hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    tape_mcap = (int)std::min<long long>(mcap, std::max(4096LL, 2LL * m + 1024));
+    return alloc_tape_mirrors();
+  }
+
+  void tape_mirrors(MirrorTape& M) {
+    const dim3 B(mps::kBlock);
+    hipLaunchKernelGGL(mps::k_inverse, dim3(nblk(nf)), B, 0, stream, (const int*)fid[0], nf, inv);
+    hipLaunchKernelGGL((mps::k_tape_mirrors<real, DIM>), dim3(nblk(tape_mcap, 2048)), B, 0, stream,
+                       (const V*)mpos, (const int*)mid, (const int*)(mcs + tot), (const int*)inv, tape_mcap,
+                       M.pos, M.par, oflow);
+    d2d(M.cs, mcs, (size_t)tot + 2);
+    d2d(M.near, mnear, (size_t)tot);
+  }
+
+  // One step with unfused phase kernels (bit-identical to the fused step)
+  // that records what the reverse sweep needs.
+  void taped_step(TapeSlot& T) {
+    const dim3 B(mps::kBlock), G(nblk(nf));
+    acc_live = true;
+    bucket();
+    d2d(T.perm1, perm, nf);
+    d2d(T.fcs1, fcs, (size_t)tot + 2);
+    d2d(T.x, fpos[0], nf);
+    d2d(T.v, fvel[0], nf);
+    mirror();
+    tape_mirrors(T.A);
+    run_vsc(false);
+    d2d(T.a1, facc[0], nf);
+    hipLaunchKernelGGL((mps::k_up1<real, DIM>), G, B, 0, stream, fpos[0], fvel[0], facc[0], ftyp[0], fprs[0],
+                       (const int*)fsid[0], (const mps::Cst<real>*)dtab, nf);
+    d2d(T.typ1, ftyp[0], nf);
+    bucket();
+    d2d(T.perm2, perm, nf);
+    d2d(T.fcs2, fcs, (size_t)tot + 2);
+    d2d(T.x1, fpos[0], nf);
+    d2d(T.v1, fvel[0], nf);
+    run_nlist();
+    d2d(T.nlist, nlist, (size_t)mps::kNList * nf);
+    d2d(T.ncount, ncount, nf);
+    run_col(true);
+    d2d(T.v2, fvel[0], nf);
+    run_prs();
+    d2d(T.P, fprs[0], nf);
+    d2d(T.nd, fnd[0], nf);
+    mirror();
+    tape_mirrors(T.B);
+    run_grad(false);
+    d2d(T.a2, facc[0], nf);
+    hipLaunchKernelGGL((mps::k_up2<real, DIM>), G, B, 0, stream, fpos[0], fvel[0], facc[0], ftyp[0], fprs[0],
+                       fpav[0], (const int*)fsid[0], (const mps::Cst<real>*)dtab, nf);
+    d2d(T.typ2, ftyp[0], nf);
+    if (wall_pav && nw > 0)
+      hipLaunchKernelGGL((mps::k_wall_pav<real>), dim3(nblk(nw)), B, 0, stream, wpav, wprs, nw);
+  }
+
+  // Reverse of one step: (xbA, vbA) = adjoints of the step's output
+  // positions and velocities in the step's final order; on return, the
+  // adjoints of its input positions and velocities in its input order.
+  void back_step(const TapeSlot& T) {
+    const dim3 B(mps::kBlock), G(nblk(nf)), GM(nblk(tape_mcap, 4096));
+    const mps::Cst<real> c = csts[0];
+    const V zero = mps::mkvec<real, DIM>(0, 0, 0);
+    TLAUNCH("adj_up2", (mps::k_adj_up2<real, DIM>), G, B, 0, stream, (const int*)T.fcs2, tot, nf,
+            (const V*)T.v2, (const V*)T.a2, (const int8_t*)T.typ2, xbA, (const V*)vbA, abuf, gbuf, c);
+    hipLaunchKernelGGL((mps::k_fill<V>), GM, B, 0, stream, yAb, tape_mcap, zero);
+    hipLaunchKernelGGL((mps::k_fill<V>), GM, B, 0, stream, zBb, tape_mcap, zero);
+    hipLaunchKernelGGL((mps::k_fill<real>), GM, B, 0, stream, PBb, tape_mcap, real(0));
+    TLAUNCH("adj_grad_pre", (mps::k_adj_grad_pre<real, DIM>), G, B, 0, stream, (const V*)T.x1,
+            (const real*)T.P, (const int*)T.fcs2, nf, (const int*)T.nlist, (const int*)T.ncount,
+            (const V*)T.B.pos, (const int*)T.B.cs, (const uint8_t*)T.B.near, (const int*)T.B.par,
+            (const V*)abuf, c, argm, sg, pminb, xbA, zBb, PBb);
+    TLAUNCH("adj_grad", (mps::k_adj_grad<real, DIM>), G, B, 0, stream, (const V*)T.x1, (const real*)T.P,
+            (const int*)T.fcs2, nf, (const int*)T.nlist, (const int*)T.ncount, (const V*)abuf,
+            (const int*)argm, (const real*)sg, (const real*)pminb, c, xbA, Pb);
+    TLAUNCH("adj_mirror", (mps::k_adj_mirror_parent<real, DIM>), GM, B, 0, stream, (const V*)T.B.pos,
+            (const int*)(T.B.cs + tot), (const int*)T.B.par, (const V*)T.x1, (const V*)zBb,
+            (const real*)PBb, xbA, Pb);
+    hipLaunchKernelGGL((mps::k_adj_prs_pre<real>), G, B, 0, stream, (const int*)T.fcs2, tot, nf,
+                       (const real*)T.nd, (const real*)T.P, (const real*)Pb, c, hks[0].snd, nbar, cpart);
+    hipLaunchKernelGGL((mps::k_sum_into<real>), dim3(1), dim3(1024), 0, stream, (const real*)cpart,
+                       (const int*)T.fcs2, tot, acc2 + 1);
+    TLAUNCH("adj_prs", (mps::k_adj_prs<real, DIM>), G, B, 0, stream, (const V*)T.x1, (const int*)T.fcs2, nf,
+            (const int*)T.nlist, (const int*)T.ncount, (const real*)nbar, (const V*)T.A.pos,
+            (const int*)T.A.cs, (const uint8_t*)T.A.near, c, xbA, yAb);
+    TLAUNCH("adj_col", (mps::k_adj_col<real, DIM>), G, B, 0, stream, (const V*)T.x1, (const V*)T.v1,
+            (const int*)T.fcs2, nf, (const int*)T.nlist, (const int*)T.ncount, (const V*)wpos,
+            (const V*)wvel, (const int*)wcs, (const V*)T.A.pos, (const int*)T.A.cs, (const uint8_t*)wnear,
+            (const uint8_t*)T.A.near, (const V*)gbuf, c, xbA, vbB, yAb);
+    hipLaunchKernelGGL((mps::k_unpermute<V>), G, B, 0, stream, (const int*)T.perm2, nf, (const V*)xbA, xbB);
+    hipLaunchKernelGGL((mps::k_unpermute<V>), G, B, 0, stream, (const int*)T.perm2, nf, (const V*)vbB, vbA);
+    TLAUNCH("adj_up1", (mps::k_adj_up1<real, DIM>), G, B, 0, stream, (const int*)T.fcs1, tot, nf,
+            (const V*)T.v, (const V*)T.a1, (const int8_t*)T.typ1, (const V*)xbB, (const V*)vbA, c, xbA, vbB,
+            abuf);
+    TLAUNCH("adj_vsc", (mps::k_adj_vsc<real, DIM>), G, B, 0, stream, (const V*)T.x, (const V*)T.v,
+            (const int*)T.fcs1, (const V*)T.A.pos, (const int*)T.A.cs, (const uint8_t*)T.A.near,
+            (const V*)abuf, c, real(1) / hks[0].n0, xbA, vbB, yAb, nupart);
+    hipLaunchKernelGGL((mps::k_sum_into<real>), dim3(1), dim3(1024), 0, stream, (const real*)nupart,
+                       (const int*)T.fcs1, tot, acc2);
+    hipLaunchKernelGGL((mps::k_adj_mirror_parent<real, DIM>), GM, B, 0, stream, (const V*)T.A.pos,
+                       (const int*)(T.A.cs + tot), (const int*)T.A.par, (const V*)T.x, (const V*)yAb,
+                       (const real*)nullptr, xbA, (real*)nullptr);
+    hipLaunchKernelGGL((mps::k_unpermute<V>), G, B, 0, stream, (const int*)T.perm1, nf, (const V*)xbA, xbB);
+    hipLaunchKernelGGL((mps::k_unpermute<V>), G, B, 0, stream, (const int*)T.perm1, nf, (const V*)vbB, vbA);
+    std::swap(xbA, xbB);
+  }
+
+  int adj_backward(const double* xbar, const double* vbar, double* gx0, double* gv0, double* gp) override {
+    if (!adj_ready) return set_err(MPS_E_STATE, "mps_adjoint_forward has not run");
+    drop_graph();
+    int rc;
+    if (tape.empty() && (rc = alloc_adjoint_work())) return rc;
+    const size_t n3 = 3 * (size_t)n_total;
+    const dim3 B(mps::kBlock), G(nblk(nf));
+    restore(ck_end);
+    if (xbar) HIP_TRY(hipMemcpyAsync(seed_x, xbar, n3 * sizeof(double), hipMemcpyHostToDevice, stream));
+    if (vbar) HIP_TRY(hipMemcpyAsync(seed_v, vbar, n3 * sizeof(double), hipMemcpyHostToDevice, stream));
+    hipLaunchKernelGGL((mps::k_adj_seed<real, DIM>), G, B, 0, stream, (const int*)fid[0], (const int8_t*)ftyp[0],
+                       nf, (long long)n_total, xbar ? (const double*)seed_x : nullptr,
+                       vbar ? (const double*)seed_v : nullptr, xbA, vbA);
+    hipLaunchKernelGGL((mps::k_fill<double>), dim3(1), B, 0, stream, acc2, 2, 0.0);
+    const int nck = (int)ckpts.size();
+    for (int k = nck - 1; k >= 0; k--) {
+      const int L = std::min(adj_every, adj_steps - k * adj_every);
+      for (;;) {
+        restore(ckpts[k]);
+        hipLaunchKernelGGL(mps::k_fill_int, dim3(1), B, 0, stream, oflow, 1, 0);
+        for (int t = 0; t < L; t++) taped_step(tape[t]);
+        int of = 0;
+        HIP_TRY(hipMemcpyAsync(&of, oflow, sizeof(int), hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        if (!of) break;
+        if (tape_mcap >= mcap) return set_err(MPS_E_STATE, "mirror tape overflow at full capacity");
+        tape_mcap = (int)std::min<long long>(mcap, 2LL * tape_mcap);
+        if ((rc = alloc_tape_mirrors())) return rc;
+      }
+      for (int t = L - 1; t >= 0; t--) back_step(tape[t]);
+    }
+    hipLaunchKernelGGL((mps::k_fill<double>), dim3(nblk(n3, 4096)), B, 0, stream, gout_x, (int)n3, 0.0);
+    hipLaunchKernelGGL((mps::k_fill<double>), dim3(nblk(n3, 4096)), B, 0, stream, gout_v, (int)n3, 0.0);
+    hipLaunchKernelGGL((mps::k_adj_out<real, DIM>), G, B, 0, stream, (const int*)ckpts[0].id, nf,
+                       (long long)n_total, (const V*)xbA, (const V*)vbA, gout_x, gout_v);
+    if (gx0) HIP_TRY(hipMemcpyAsync(gx0, gout_x, n3 * sizeof(double), hipMemcpyDeviceToHost, stream));
+    if (gv0) HIP_TRY(hipMemcpyAsync(gv0, gout_v, n3 * sizeof(double), hipMemcpyDeviceToHost, stream));
+    if (gp) HIP_TRY(hipMemcpyAsync(gp, acc2, 2 * sizeof(double), hipMemcpyDeviceToHost, stream));
+    restore(ck_end);
+    HIP_TRY(hipStreamSynchronize(stream));
+    return check_launch();
+  }
+
   int constants(int scene, mps_constants* out) override {
     if (scene < 0 || scene >= S) return set_err(MPS_E_ARG, "scene index out of range");
     mps::fill_constants_out(hks[scene], DIM, out);
@@ -1030,6 +1353,23 @@ int mps_set_state(mps_handle* h, const int8_t* typ, const double* pos, const dou
 int mps_get_buckets(mps_handle* h, int which, int* cell_start, int* ids, int* count) {
   if (!h || !count || which < 0 || which > 2) return set_err(MPS_E_ARG, "bad argument");
   return h->impl->buckets(which, cell_start, ids, count);
+}
+
+int mps_adjoint_forward(mps_handle* h, int nsteps, int checkpoint_every) {
+  if (!h) return set_err(MPS_E_ARG, "null handle");
+  return h->impl->adj_forward(nsteps, checkpoint_every);
+}
+
+int mps_adjoint_backward(mps_handle* h, const double* pos_bar, const double* vel_bar, double* pos0_bar,
+                         double* vel0_bar, double* param_bar) {
+  if (!h) return set_err(MPS_E_ARG, "null handle");
+  return h->impl->adj_backward(pos_bar, vel_bar, pos0_bar, vel0_bar, param_bar);
+}
+
+int mps_adjoint_release(mps_handle* h) {
+  if (!h) return set_err(MPS_E_ARG, "null handle");
+  h->impl->adj_release();
+  return MPS_OK;
 }
 
 int mps_get_mirrors(mps_handle* h, int* ids, double* pos, double* prs, int* count) {

@@ -93,8 +93,12 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     for fn in ("mps_scene_dambreak_count", "mps_scene_dambreak_fill", "mps_params_default",
                "mps_compute_constants", "mps_compute_edge", "mps_probe_arith", "mps_sync",
                "mps_set_stream", "mps_timing_enable", "mps_timing_read", "mps_create_batch",
-               "mps_get_scene_constants", "mps_set_graph", "mps_set_launch"):
+               "mps_get_scene_constants", "mps_set_graph", "mps_set_launch",
+               "mps_adjoint_forward", "mps_adjoint_backward", "mps_adjoint_release"):
         getattr(lib, fn).restype = C.c_int
+    lib.mps_adjoint_forward.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    lib.mps_adjoint_backward.argtypes = [C.c_void_p] * 6
+    lib.mps_adjoint_release.argtypes = [C.c_void_p]
     lib.mps_create_batch.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]
     lib.mps_get_scene_constants.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
     lib.mps_set_graph.argtypes = [C.c_void_p, C.c_int]
@@ -279,6 +283,26 @@ class Handle:
 
     def set_launch(self, block: int, wave_grid: int = 16384):
         self._check(self.lib.mps_set_launch(self._h, block, wave_grid))
+
+    def adjoint_forward(self, nsteps: int, checkpoint_every: int):
+        """nsteps steps keeping checkpoints for adjoint_backward."""
+        self._check(self.lib.mps_adjoint_forward(self._h, nsteps, checkpoint_every))
+
+    def adjoint_backward(self, pos_bar=None, vel_bar=None) -> dict:
+        """Reverse sweep for seeds dL/dpos, dL/dvel of the final state ([3, n]).
+
+        Returns {"pos0", "vel0": [3, n] adjoints of the initial state,
+        "nu", "c": dL/dnu, dL/dc}."""
+        n = self.n
+        seeds = [None if a is None else np.ascontiguousarray(a, np.float64).reshape(3, n)
+                 for a in (pos_bar, vel_bar)]
+        gx, gv, gp = np.zeros((3, n)), np.zeros((3, n)), np.zeros(2)
+        self._check(self.lib.mps_adjoint_backward(self._h, *(_ptr(a) for a in seeds),
+                                                  gx.ctypes.data, gv.ctypes.data, gp.ctypes.data))
+        return {"pos0": gx, "vel0": gv, "nu": float(gp[0]), "c": float(gp[1])}
+
+    def adjoint_release(self):
+        self._check(self.lib.mps_adjoint_release(self._h))
 
     def set_stream(self, stream: int):
         """hipStream_t as an integer (e.g. torch.cuda.Stream.cuda_stream); 0 = own."""

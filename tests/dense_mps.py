@@ -1,0 +1,180 @@
+"""Dense O(N^2) PyTorch restatement of the explicit MPS step, for checking
+the adjoint with autograd.  TEST INFRASTRUCTURE ONLY.
+
+Every pair interaction is evaluated on full N x N (fluid), N x walls and
+N x mirrors matrices with the step's own radius tests as masks, in float64 on
+the CPU, so autograd gives the derivative of the step with its discrete
+choices frozen -- the quantity the hand-written adjoint computes.  The mirror
+sets (which parent, which wall plane, which mirrors survive the cull) are not
+re-derived here: they are recorded from a forward run (GPU handle or CPU
+oracle) and each mirror is rebuilt as the reflection of its parent across the
+recorded plane, y = x - 2 (x . n - off) n.
+
+Follows the phase formulas of kernel.cu: VscTrm 1013-1119, UpPcl1 1138-1149,
+ChkCol 1168-1287, MkPrs 1306-1411, PrsGrdTrm 1651-1794, UpPcl2 1815-1829,
+ChkPcl 481-513 (speed clamp; no particle may leave the domain here).
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from mps_cuda_amd import ffi
+
+_F64 = torch.float64
+
+
+class Plane:
+    """Frozen mirror set of one mirror phase: parent index (into the fluid
+    list), unit plane normal and plane offset per mirror."""
+
+    def __init__(self, par, nhat, off):
+        self.par = torch.as_tensor(par, dtype=torch.long)
+        self.nhat = torch.as_tensor(nhat, dtype=_F64)
+        self.off = torch.as_tensor(off, dtype=_F64)
+
+    def positions(self, x):
+        xp = x[self.par]
+        return xp - 2.0 * ((xp * self.nhat).sum(-1) - self.off)[:, None] * self.nhat
+
+
+def _plane_from(handle, local_of_id, dim):
+    ids, mpos, _ = handle.mirrors()
+    st = handle.state()
+    par_id = ids // 10
+    par = np.array([local_of_id[i] for i in par_id], dtype=np.int64)
+    xp = st["pos"][:dim, par_id].T
+    y = mpos[:dim].T
+    nv = xp - y
+    nn = np.linalg.norm(nv, axis=1, keepdims=True)
+    nhat = nv / np.where(nn > 0, nn, 1.0)
+    off = (nhat * 0.5 * (xp + y)).sum(1)
+    return Plane(par, nhat, off)
+
+
+def record_planes(handle, nsteps, fluid_ids, dim):
+    """Steps `handle` phase by phase and records the two mirror sets of each
+    step.  Returns [(planes_A, planes_B)] per step."""
+    local_of_id = {int(i): k for k, i in enumerate(fluid_ids)}
+    out = []
+    for _ in range(nsteps):
+        rec = []
+        for ph in ffi.STEP_PHASES:
+            handle.phase(ph)
+            if ph == ffi.PH_MIRROR:
+                rec.append(_plane_from(handle, local_of_id, dim))
+        out.append(tuple(rec))
+    return out
+
+
+class DenseMPS:
+    """The step on dense matrices.  `consts` is ffi.compute_constants()."""
+
+    def __init__(self, params, scene, consts):
+        self.dim = params.dim
+        d = self.dim
+        typ = scene.typ
+        self.fluid = np.where(typ == ffi.FLD)[0]
+        walls = np.where((typ == ffi.WLL) | (typ == ffi.OBJ) | (typ == ffi.OBJ2))[0]
+        self.w = torch.as_tensor(scene.pos[:d, walls].T.copy(), dtype=_F64)
+        wv = scene.vel[:d, walls].T.copy() if scene.vel is not None else np.zeros((len(walls), d))
+        self.wv = torch.as_tensor(wv, dtype=_F64)
+        k = consts
+        self.l0, self.dt, self.rho = params.l0, k["dt"], params.rho
+        self.r, self.rp = k["r"], k["rp"]
+        self.rp2 = self.rp * self.rp
+        self.rlim2, self.rlimw2 = k["rlim"] ** 2, k["rlim_wall"] ** 2
+        self.n0, self.n0g = k["n0"], k["n0_grad"]
+        self.umax, self.pmax, self.col = k["umax"], k["pmax"], k["col"]
+        self.g = torch.as_tensor(list(params.gravity)[:d], dtype=_F64)
+
+    def initial(self, scene):
+        d = self.dim
+        x = torch.as_tensor(scene.pos[:d, self.fluid].T.copy(), dtype=_F64)
+        v = scene.vel[:d, self.fluid].T.copy() if scene.vel is not None else np.zeros_like(x.numpy())
+        return x, torch.as_tensor(v, dtype=_F64)
+
+    @staticmethod
+    def _pairs(xa, xb):
+        p = xb[None, :, :] - xa[:, None, :]
+        return p, (p * p).sum(-1)
+
+    def _clamp(self, u):
+        U = torch.sqrt((u * u).sum(-1, keepdim=True))
+        return torch.where(U > self.umax, u * (self.umax / U), u)
+
+    def step(self, x, v, nu, c, planes):
+        """One step; x, v: [N, dim]; nu, c: 0-d tensors; planes = (A, B)."""
+        A, B = planes
+        N = x.shape[0]
+        eye = torch.eye(N, dtype=torch.bool)
+        one = torch.ones((), dtype=_F64)
+        yA = A.positions(x)
+        # VscTrm (mirror velocity is 0)
+        p, d2 = self._pairs(x, x)
+        dist = torch.sqrt(torch.where(eye, one, d2))
+        m = (~eye) & (dist <= self.r)
+        h = torch.where(m, 2.0 * self.l0 / dist ** 3, 0.0)
+        S = (h[..., None] * (v[None, :, :] - v[:, None, :])).sum(1)
+        pm, dm2 = self._pairs(x, yA)
+        dm = torch.sqrt(dm2)
+        hm = torch.where(dm <= self.r, 2.0 * self.l0 / dm ** 3, 0.0)
+        S = S - hm.sum(1)[:, None] * v
+        a1 = (nu / self.n0) * S + self.g
+        # UpPcl1: position with the unclamped velocity, then the clamp
+        vu = v + a1 * self.dt
+        x1 = x + vu * self.dt
+        v1 = self._clamp(vu)
+        # ChkCol + UpChkCol (Jacobi: every pair uses the pre-collision v1)
+        kap = self.col * 0.5
+        p, d2 = self._pairs(x1, x1)
+        u = v1[:, None, :] - v1[None, :, :]
+        fdt = (u * p).sum(-1)
+        act = (~eye) & (d2 < self.rlim2) & (fdt > 0)
+        corr = (torch.where(act, kap * fdt / torch.where(eye, one, d2), 0.0)[..., None] * p).sum(1)
+        pw, dw2 = self._pairs(x1, self.w)
+        uw = v1[:, None, :] - self.wv[None, :, :]
+        fw = (uw * pw).sum(-1)
+        actw = (dw2 < self.rlimw2) & (fw > 0)
+        corr = corr + (torch.where(actw, kap * fw / dw2, 0.0)[..., None] * pw).sum(1)
+        pm, dm2 = self._pairs(x1, yA)
+        fm = (v1[:, None, :] * pm).sum(-1)
+        actm = (dm2 < self.rlim2) & (fm > 0)
+        corr = corr + (torch.where(actm, kap * fm / dm2, 0.0)[..., None] * pm).sum(1)
+        v2 = v1 - corr
+        # MkPrs (fluid + A mirrors, sqrt(d2) < rp)
+        dist = torch.sqrt(torch.where(eye, one, d2))
+        mp = (~eye) & (dist < self.rp)
+        n = torch.where(mp, self.rp / dist - dist / self.rp, 0.0).sum(1)
+        dmA = torch.sqrt(dm2)
+        n = n + torch.where(dmA < self.rp, self.rp / dmA - dmA / self.rp, 0.0).sum(1)
+        coef = c * c / self.n0g * self.rho
+        P = torch.where(n > self.n0g, (n - self.n0g) * coef, torch.zeros_like(n))
+        P = torch.where(P > self.pmax, torch.full_like(P, self.pmax), P)
+        # mirrors B from the moved positions and the new pressure
+        yB = B.positions(x1)
+        PB = P[B.par]
+        # PrsGrdTrm (d2 < rp2); P_min over the particle and its fluid neighbours
+        mg = (~eye) & (d2 < self.rp2)
+        inf = torch.full((N, N), float("inf"), dtype=_F64)
+        pmin = torch.minimum(P, torch.where(mg, P[None, :].expand(N, N), inf).min(1).values)
+        d2s = torch.where(eye, one, d2)
+        ds = torch.sqrt(d2s)
+        f = torch.where(mg, (self.rp / ds - ds / self.rp) / d2s, 0.0)
+        acc = ((P[None, :] - pmin[:, None]) * f)[..., None] * p
+        acc = acc.sum(1)
+        pb, db2 = self._pairs(x1, yB)
+        mb = db2 < self.rp2
+        db = torch.sqrt(db2)
+        fb = torch.where(mb, (self.rp / db - db / self.rp) / db2, 0.0)
+        acc = acc + (((PB[None, :] - pmin[:, None]) * fb)[..., None] * pb).sum(1)
+        a2 = -(1.0 / self.rho) * (self.dim / self.n0g) * acc
+        # UpPcl2
+        vw = v2 + a2 * self.dt
+        x2 = x1 + a2 * self.dt * self.dt
+        return x2, self._clamp(vw)
+
+    def rollout(self, x, v, nu, c, plane_seq):
+        for planes in plane_seq:
+            x, v = self.step(x, v, nu, c, planes)
+        return x, v

@@ -1,0 +1,64 @@
+"""The dense PyTorch step (the adjoint's autograd checker) against the CPU
+oracle: with the oracle's mirror sets it must reproduce the oracle's
+trajectory, and its autograd must agree with central differences."""
+import numpy as np
+import torch
+
+from mps_cuda_amd import configs, ffi
+from tests.dense_mps import DenseMPS, record_planes
+
+
+def _setup(oracle_mod, dim, steps):
+    if dim == 2:
+        sc, p = configs.small2d("fp64", jitter=0.01, seed=3)
+    else:
+        sc, p = configs.small3d("fp64", seed=3)
+    sc = configs.with_velocity(sc, 0.02, 11, dim)
+    o = oracle_mod.Oracle(p, sc)
+    dm = DenseMPS(p, sc, ffi.compute_constants(p, sc))
+    planes = record_planes(o, steps, dm.fluid, dim)
+    return sc, p, o, dm, planes
+
+
+def test_dense_step_reproduces_the_oracle_2d(oracle_mod):
+    sc, p, o, dm, planes = _setup(oracle_mod, 2, 30)
+    x, v = dm.initial(sc)
+    xT, vT = dm.rollout(x, v, torch.tensor(p.nu, dtype=torch.float64),
+                        torch.tensor(p.c, dtype=torch.float64), planes)
+    st = o.state()
+    assert np.max(np.abs(xT.numpy() - st["pos"][:2, dm.fluid].T)) < 1e-13
+    assert np.max(np.abs(vT.numpy() - st["vel"][:2, dm.fluid].T)) < 1e-11
+
+
+def test_dense_step_reproduces_the_oracle_3d(oracle_mod):
+    sc, p, o, dm, planes = _setup(oracle_mod, 3, 4)
+    x, v = dm.initial(sc)
+    xT, vT = dm.rollout(x, v, torch.tensor(p.nu, dtype=torch.float64),
+                        torch.tensor(p.c, dtype=torch.float64), planes)
+    st = o.state()
+    assert np.max(np.abs(xT.numpy() - st["pos"][:3, dm.fluid].T)) < 1e-13
+    assert np.max(np.abs(vT.numpy() - st["vel"][:3, dm.fluid].T)) < 1e-11
+
+
+def test_dense_autograd_agrees_with_central_differences(oracle_mod):
+    sc, p, o, dm, planes = _setup(oracle_mod, 2, 20)
+    x, v = dm.initial(sc)
+    nu0 = torch.tensor(p.nu, dtype=torch.float64)
+    c0 = torch.tensor(p.c, dtype=torch.float64)
+    target = dm.rollout(x, v, nu0, c0, planes)[0].detach() + 0.002
+
+    def loss(nu, c, vv):
+        return ((dm.rollout(x, vv, nu, c, planes)[0] - target) ** 2).sum(-1).mean()
+
+    c_t = c0.clone().requires_grad_(True)
+    v_t = v.clone().requires_grad_(True)
+    loss(nu0, c_t, v_t).backward()
+    u = torch.randn(v.shape, dtype=torch.float64, generator=torch.Generator().manual_seed(2))
+    u /= u.norm()
+    with torch.no_grad():
+        hc = 1e-5 * p.c
+        fdc = (loss(nu0, c0 + hc, v) - loss(nu0, c0 - hc, v)) / (2 * hc)
+        hv = 1e-5
+        fdv = (loss(nu0, c0, v + hv * u) - loss(nu0, c0, v - hv * u)) / (2 * hv)
+    assert abs(fdc.item() / c_t.grad.item() - 1) < 1e-4
+    assert abs(fdv.item() / (v_t.grad * u).sum().item() - 1) < 1e-4
