@@ -17,6 +17,8 @@ before the timed region).  Prints one JSON line:
                 handle, launch shape autotuned, steps replayed from a
                 hipGraph, against the same scenes as single-scene handles
                 stepped one after another with plain launches ("speedup")
+  adjoint       the 300k scene for 200 steps with checkpoints every 25:
+                (forward with checkpoints + reverse sweep) / plain forward
 
 Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-batch]
 """
@@ -162,6 +164,52 @@ def batched_leg(steps: int, seq_steps: int, log) -> dict:
             "autotune": {k: tuned[k] for k in ("block", "wave_grid", "graph", "trials")}}
 
 
+def adjoint_leg(steps: int, every: int, log) -> dict:
+    """cfg4 timing case: cfg1 (300k fluid, FP32), `steps` steps with a
+    checkpoint every `every` steps, forward + reverse sweep, against the
+    same steps run plainly."""
+    from mps_cuda_amd import configs, ffi
+
+    scene, params = configs.cfg1("fp32")
+    plain = ffi.Handle(params, scene)
+    plain.step(2)
+    plain.sync()
+    plain.set_state(scene.typ, scene.pos, np.zeros((3, scene.n)), np.zeros(scene.n), np.zeros(scene.n))
+    t0 = time.perf_counter()
+    plain.step(steps)
+    plain.sync()
+    t_plain = time.perf_counter() - t0
+    plain.close()
+
+    h = ffi.Handle(params, scene)
+    fl = scene.typ == ffi.FLD
+
+    def sweep():
+        t0 = time.perf_counter()
+        h.adjoint_forward(steps, every)
+        h.sync()
+        t1 = time.perf_counter()
+        st = h.state()  # loss gradient on the host (not timed)
+        bar = np.zeros((3, h.n))
+        bar[:, fl] = 2.0 * (st["pos"][:, fl] - scene.pos[:, fl]) / fl.sum()
+        t2 = time.perf_counter()
+        g = h.adjoint_backward(bar)
+        t3 = time.perf_counter()
+        return t1 - t0, t3 - t2, g
+
+    sweep()  # first sweep allocates checkpoints and tapes
+    h.set_state(scene.typ, scene.pos, np.zeros((3, scene.n)), np.zeros(scene.n), np.zeros(scene.n))
+    t_f, t_b, g = sweep()
+    h.close()
+    log(f"[bench] adjoint: plain {t_plain:.3f} s, forward+checkpoints {t_f:.3f} s, reverse {t_b:.3f} s")
+    return {"workload": f"cfg4 timing case: cfg1 3-D dam break, 300k fluid, FP32, {steps} steps, "
+                        f"checkpoint every {every}",
+            "ratio": round((t_f + t_b) / t_plain, 3), "unit": "adjoint time / forward time",
+            "forward_s": round(t_plain, 4), "forward_checkpointed_s": round(t_f, 4),
+            "reverse_s": round(t_b, 4),
+            "grad": {"nu": g["nu"], "c": g["c"], "vel0_norm": float(np.linalg.norm(g["vel0"]))}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -172,6 +220,9 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=200)
     ap.add_argument("--seq-steps", type=int, default=10)
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-steps", type=int, default=200)
+    ap.add_argument("--adjoint-every", type=int, default=25)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -247,6 +298,11 @@ def main():
         batch = batched_leg(args.batch_steps, args.seq_steps,
                             lambda msg: print(msg, file=sys.stderr, flush=True))
 
+    adj = None
+    if not args.no_adjoint:
+        adj = adjoint_leg(args.adjoint_steps, args.adjoint_every,
+                          lambda msg: print(msg, file=sys.stderr, flush=True))
+
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "particle-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * elapsed / args.steps, 4),
@@ -258,6 +314,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "batched": batch,
+        "adjoint": adj,
         "kernels_ms": {k: round(v[0] / v[1], 5) for k, v in sorted(kt.items(), key=lambda kv: -kv[1][0])},
     }
     print(json.dumps(out), flush=True)

@@ -928,7 +928,8 @@ struct Impl : ImplBase {
     int8_t* typ;
     int *id, *sid;
   };
-  std::vector<void*> adj_allocs, adj_mallocs;  // fixed-size / mirror-sized adjoint memory
+  // adjoint memory: checkpoints / tapes and work arrays / mirror-sized arrays
+  std::vector<void*> ck_allocs, adj_allocs, adj_mallocs;
   std::vector<Ckpt> ckpts;
   Ckpt ck_end{};
   std::vector<TapeSlot> tape;
@@ -954,15 +955,23 @@ struct Impl : ImplBase {
     for (void* p : l) (void)hipFree(p);
     l.clear();
   }
-  void adj_release() override {
+  void free_tape() {
     if (!adj_allocs.empty() || !adj_mallocs.empty()) (void)hipStreamSynchronize(stream);
     free_list(adj_mallocs);
     free_list(adj_allocs);
-    ckpts.clear();
     tape.clear();
-    ck_end = Ckpt{};
-    adj_ready = false;
     tape_mcap = 0;
+  }
+  void free_ckpts() {
+    if (!ck_allocs.empty()) (void)hipStreamSynchronize(stream);
+    free_list(ck_allocs);
+    ckpts.clear();
+    ck_end = Ckpt{};
+  }
+  void adj_release() override {
+    free_tape();
+    free_ckpts();
+    adj_ready = false;
   }
 
   template <class T>
@@ -972,11 +981,11 @@ struct Impl : ImplBase {
 
   int alloc_ckpt(Ckpt& k) {
     int rc;
-    if ((rc = aalloc(adj_allocs, &k.pos, nf)) || (rc = aalloc(adj_allocs, &k.vel, nf)) ||
-        (rc = aalloc(adj_allocs, &k.prs, nf)) || (rc = aalloc(adj_allocs, &k.pav, nf)) ||
-        (rc = aalloc(adj_allocs, &k.nd, nf)) || (rc = aalloc(adj_allocs, &k.wpav, nw)) ||
-        (rc = aalloc(adj_allocs, &k.typ, nf)) || (rc = aalloc(adj_allocs, &k.id, nf)) ||
-        (rc = aalloc(adj_allocs, &k.sid, nf)))
+    if ((rc = aalloc(ck_allocs, &k.pos, nf)) || (rc = aalloc(ck_allocs, &k.vel, nf)) ||
+        (rc = aalloc(ck_allocs, &k.prs, nf)) || (rc = aalloc(ck_allocs, &k.pav, nf)) ||
+        (rc = aalloc(ck_allocs, &k.nd, nf)) || (rc = aalloc(ck_allocs, &k.wpav, nw)) ||
+        (rc = aalloc(ck_allocs, &k.typ, nf)) || (rc = aalloc(ck_allocs, &k.id, nf)) ||
+        (rc = aalloc(ck_allocs, &k.sid, nf)))
       return rc;
     return MPS_OK;
   }
@@ -1011,14 +1020,18 @@ struct Impl : ImplBase {
     if (nsteps < 1 || every < 1) return set_err(MPS_E_ARG, "need nsteps >= 1 and checkpoint_every >= 1");
     if (nf == 0) return set_err(MPS_E_STATE, "no fluid particles");
     drop_graph();
-    adj_release();
+    adj_ready = false;
     step_eager(0);  // clears a phase-mode acceleration
     const int nck = (nsteps + every - 1) / every;
-    ckpts.resize(nck);
     int rc;
-    for (auto& k : ckpts)
-      if ((rc = alloc_ckpt(k))) return rc;
-    if ((rc = alloc_ckpt(ck_end))) return rc;
+    if (every != adj_every) free_tape();  // tape slots are sized by `every`
+    if ((int)ckpts.size() != nck) {       // memory is kept across sweeps of one shape
+      free_ckpts();
+      ckpts.resize(nck);
+      for (auto& k : ckpts)
+        if ((rc = alloc_ckpt(k))) return rc;
+      if ((rc = alloc_ckpt(ck_end))) return rc;
+    }
     for (int k = 0; k < nck; k++) {
       save(ckpts[k]);
       step_eager(std::min(every, nsteps - k * every));
