@@ -37,6 +37,24 @@ def hipcc() -> str:
     return "hipcc"
 
 
+DRIVER = os.path.join(HERE, "mps_run")
+
+
+def build_driver(verbose: bool = False) -> str:
+    """The command-line driver (host C++ over the C ABI, linked to the library)."""
+    tmp = DRIVER + ".tmp"
+    cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"),
+           os.path.join(CSRC, "mps_run.cpp"), "-L", HERE, "-lmps_hip", "-Wl,-rpath,$ORIGIN", "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("building mps_run failed")
+    os.replace(tmp, DRIVER)
+    return DRIVER
+
+
 def build(verbose: bool = False) -> str:
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
     tmp = LIB + ".tmp"
@@ -48,6 +66,7 @@ def build(verbose: bool = False) -> str:
         sys.stderr.write(res.stdout + res.stderr)
         raise RuntimeError("hipcc failed building libmps_hip.so")
     os.replace(tmp, LIB)
+    build_driver(verbose)
     return LIB
 
 

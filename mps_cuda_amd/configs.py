@@ -6,6 +6,8 @@
         0.25 x 0.5 x 0.3 (300,000 fluid), jitter +-0.01 l0 (seed 1),
         dt = 2.5e-4, 2000 steps.
   cfg2a 3-D dam break at l0 = 0.0025 (2.4 M fluid), 500 steps.
+  cfg2b 2.4 M fluid particles uniform at random (seed 2) in a 0.5^3 m box, no
+        walls, l0 = (0.125 / 2.4e6)^(1/3) = 3.735 mm (mean density n0), one step.
   cfg3  512 independent 2-D dam breaks, FP32, l0 = 0.005, tank 1.0 x 0.6;
         per scene (seed 3): column width U[0.15, 0.35], height U[0.3, 0.6]
         (1.8k-8.4k fluid), nu log-uniform in [1e-6, 1e-4], c U[15, 30];
@@ -55,6 +57,18 @@ def cfg2a(precision="fp32", **kw):
     sc = ffi.dambreak(dim=3, l0=0.0025, tank=(1.0, 0.6, 0.3), column=(0.25, 0.5, 0.3),
                       wall_layers=3, jitter=0.01, seed=2, domain_top=1.0)
     return sc, params_for(sc, 3, 0.0025, 1.25e-4, precision, **kw)
+
+
+def cfg2b(precision="fp32", n=2_400_000, seed=2, **kw):
+    """Uniform random placement: uneven bucket occupancy (Poisson, mean ~30)."""
+    rng = np.random.default_rng(seed)
+    side = 0.5
+    l0 = (side ** 3 / n) ** (1.0 / 3.0)
+    pos = np.zeros((3, n))
+    pos[:] = rng.uniform(0.0, side, (3, n))
+    sc = ffi.SceneArrays(typ=np.zeros(n, np.int8), pos=pos, normal=np.zeros((3, n)),
+                         range_min=(0.0, 0.0, 0.0), range_max=(side, side, side))
+    return sc, params_for(sc, 3, l0, 1.25e-4, precision, **kw)
 
 
 def small3d(precision="fp64", l0=0.02, jitter=0.01, seed=5, **kw):

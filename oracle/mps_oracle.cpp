@@ -859,7 +859,9 @@ struct Sim : Base {
     for (int c = 0; c < nBxyz; c++) {
       if (cs) cs[c] = k;
       for (int j = head[c]; j != -1; j = next[j]) {
-        bool take = mir || (which == MPS_BKT_FLUID ? Typ[j] == FLD : is_wall(Typ[j]));
+        // the lists hold what was FLD or wall at the last build; a particle
+        // deleted since (UpPcl2) stays in them until the next MkBkt
+        bool take = mir || (which == MPS_BKT_FLUID ? !is_wall(Typ[j]) : is_wall(Typ[j]));
         if (!take) continue;
         if (ids) ids[k] = j;
         k++;
