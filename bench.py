@@ -210,36 +210,9 @@ def adjoint_leg(steps: int, every: int, log) -> dict:
             "grad": {"nu": g["nu"], "c": g["c"], "vel0_norm": float(np.linalg.norm(g["vel0"]))}}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-batch", action="store_true")
-    ap.add_argument("--batch-steps", type=int, default=200)
-    ap.add_argument("--seq-steps", type=int, default=10)
-    ap.add_argument("--no-adjoint", action="store_true")
-    ap.add_argument("--adjoint-steps", type=int, default=200)
-    ap.add_argument("--adjoint-every", type=int, default=25)
-    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
-                    help="only run a plain device copy of MIB MiB (calibrates HBM byte counters)")
-    args = ap.parse_args()
-    if args.copy_probe:
-        import torch
-        a = torch.empty(args.copy_probe << 18, dtype=torch.float32, device="cuda")
-        b = torch.empty_like(a)
-        for _ in range(3):
-            b.copy_(a)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_probe_bytes_each_way": a.numel() * 4, "launches": 3}))
-        return
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU only: this build has no multi-GPU path (asked for {args.gpus})",
-              file=sys.stderr)
-        sys.exit(2)
-
+def plain_leg(args) -> dict:
+    """cfg1: timed steps (event-free), then a deterministic replay with
+    per-kernel HIP events for the roofline."""
     from mps_cuda_amd import configs, ffi
 
     scene, params = configs.cfg1("fp32")
@@ -287,6 +260,41 @@ def main():
                 "bytes_per_launch": bytes_, "avg_ms": round(avg_ms, 5),
                 "valu": {"achieved": round(tflops, 2), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                          "frac": round(tflops / FP32_PEAK_TFLOPS, 4), "flops_per_launch": flops}}
+    return dict(value=value, elapsed=elapsed, nf0=nf0, nw=nw, live=live, m=m, roofline=roofline, kt=kt)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="skip the cfg1 leg (profiling other legs)")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-steps", type=int, default=200)
+    ap.add_argument("--seq-steps", type=int, default=10)
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-steps", type=int, default=200)
+    ap.add_argument("--adjoint-every", type=int, default=25)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (calibrates HBM byte counters)")
+    args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        a = torch.empty(args.copy_probe << 18, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_each_way": a.numel() * 4, "launches": 3}))
+        return
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only: this build has no multi-GPU path (asked for {args.gpus})",
+              file=sys.stderr)
+        sys.exit(2)
+
+    plain = None if args.no_plain else plain_leg(args)
 
     cpu = None
     if not args.no_cpu:
@@ -303,19 +311,21 @@ def main():
         adj = adjoint_leg(args.adjoint_steps, args.adjoint_every,
                           lambda msg: print(msg, file=sys.stderr, flush=True))
 
+    P = plain or dict(value=None, elapsed=None, nf0=None, nw=None, live=None, m=None, roofline=None, kt={})
     out = {
-        "metric": METRIC, "value": round(value, 1), "unit": "particle-steps/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * elapsed / args.steps, 4),
+        "metric": METRIC, "value": None if plain is None else round(P["value"], 1), "unit": "particle-steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": None if plain is None else round(1e3 * P["elapsed"] / args.steps, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "cfg1: 3-D dam break, 300k fluid, FP32", "fluid": nf0, "walls": nw,
-                   "live_fluid": live, "mirrors": m, "l0": 0.005, "dt": 2.5e-4,
+        "config": {"workload": "cfg1: 3-D dam break, 300k fluid, FP32", "fluid": P["nf0"], "walls": P["nw"],
+                   "live_fluid": P["live"], "mirrors": P["m"], "l0": 0.005, "dt": 2.5e-4,
                    "tank": [1.0, 0.6, 0.3], "column": [0.25, 0.5, 0.3]},
-        "roofline": roofline,
+        "roofline": P["roofline"],
         "cpu_baseline": cpu,
         "batched": batch,
         "adjoint": adj,
-        "kernels_ms": {k: round(v[0] / v[1], 5) for k, v in sorted(kt.items(), key=lambda kv: -kv[1][0])},
+        "kernels_ms": {k: round(v[0] / v[1], 5) for k, v in sorted(P["kt"].items(), key=lambda kv: -kv[1][0])},
     }
     print(json.dumps(out), flush=True)
 
