@@ -299,6 +299,13 @@ int mps_compute_edge(const mps_params* params, const mps_scene* scene, int8_t* e
 int mps_probe_arith(int precision, const double* a, const double* b, double* sqrt_out,
                     double* div_out, int n);
 
+/* The neighbour kernels' division by a reused divisor (FP32: reciprocal
+ * refined to RN(1/b), then one correction per quotient) on n values:
+ * rcp_out = the reciprocal it uses, div_out = its a/b.  Used to show that it
+ * equals the IEEE quotient bit for bit.  No reference counterpart. */
+int mps_probe_division(int precision, const double* a, const double* b, double* rcp_out,
+                       double* div_out, int n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1456,7 +1456,7 @@ int mps_compute_edge(const mps_params* p, const mps_scene* s, int8_t* edge) {
 }  // extern "C"
 
 template <class real>
-static int probe(const double* a, const double* b, double* so, double* qo, int n) {
+static int probe(const double* a, const double* b, double* so, double* qo, int n, int fast) {
   std::vector<real> ha(n), hb(n), hs(n), hq(n);
   for (int i = 0; i < n; i++) {
     ha[i] = real(a[i]);
@@ -1471,7 +1471,7 @@ static int probe(const double* a, const double* b, double* so, double* qo, int n
   if (e == hipSuccess) e = hipMemcpy(db, hb.data(), n * sizeof(real), hipMemcpyHostToDevice);
   if (e == hipSuccess) {
     hipLaunchKernelGGL((mps::k_probe<real>), dim3(nblk(n)), dim3(mps::kBlock), 0, nullptr, da, db, ds,
-                       dq, n);
+                       dq, n, fast);
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipMemcpy(hs.data(), ds, n * sizeof(real), hipMemcpyDeviceToHost);
@@ -1492,7 +1492,14 @@ extern "C" {
 
 int mps_probe_arith(int precision, const double* a, const double* b, double* so, double* qo, int n) {
   if (!a || !b || !so || !qo || n <= 0) return set_err(MPS_E_ARG, "bad argument");
-  return precision == MPS_FP64 ? probe<double>(a, b, so, qo, n) : probe<float>(a, b, so, qo, n);
+  return precision == MPS_FP64 ? probe<double>(a, b, so, qo, n, 0) : probe<float>(a, b, so, qo, n, 0);
+}
+
+int mps_probe_division(int precision, const double* a, const double* b, double* rcp_out, double* div_out,
+                       int n) {
+  if (!a || !b || !rcp_out || !div_out || n <= 0) return set_err(MPS_E_ARG, "bad argument");
+  return precision == MPS_FP64 ? probe<double>(a, b, rcp_out, div_out, n, 1)
+                               : probe<float>(a, b, rcp_out, div_out, n, 1);
 }
 
 }  // extern "C"

@@ -127,6 +127,61 @@ __device__ inline bool scene_block(const Scenes<real>& sc, const int* __restrict
   return true;
 }
 
+// ---- division by a reused divisor -------------------------------------------
+// The neighbour terms divide several numerators by one distance b (VscTrm:
+// three components, each divided by b three times).  For FP32 this uses
+// Markstein's method (P. W. Markstein, "Computation of elementary functions
+// on the IBM RISC System/6000 processor", IBM J. Res. Develop. 34(1), 1990;
+// J.-M. Muller et al., Handbook of Floating-Point Arithmetic, 2nd ed., 2018,
+// section 4.9):
+//   y = RN(1/b) by one Newton-Raphson step e = fma(-b, y0, 1), y = fma(e, y0, y0)
+//   from the hardware reciprocal y0.  The step can miss RN(1/b) only when the
+//   significand of b is all ones, b = 2^k (2 - 2^-23); there
+//   1/b = 2^-(k+1) (1 + 2^-24 + 2^-48 + ...), whose rounding is 2^-(k+1) (1 + 2^-23).
+//   Then for each a:  q = RN(a y),  r = fma(-b, q, a) (exact),  RN(q + r y).
+// With y = RN(1/b) the result is RN(a/b), the correctly rounded quotient --
+// bit for bit what a / b gives -- provided nothing underflows or overflows:
+// `ok` holds for b in [2^-20, 2^20] and fits(a) for |a| in [2^-60, 2^60] or 0,
+// which keeps up to three chained quotients and their residuals normal.
+// The sign of a zero quotient is restored from a (b > 0 on every use).
+// FP64 keeps the plain division.
+template <class real>
+struct Divisor;
+
+template <>
+struct Divisor<float> {
+  float b, y;
+  bool ok;
+  __device__ explicit Divisor(float b_) : b(b_) {
+    ok = b >= 0x1p-20f && b <= 0x1p20f;
+    const float y0 = __builtin_amdgcn_rcpf(b);
+    const float e = __builtin_fmaf(-b, y0, 1.0f);
+    const unsigned bits = __float_as_uint(b);
+    const float yfix = __uint_as_float(((253u - ((bits >> 23) & 0xffu)) << 23) | 1u);
+    y = (bits & 0x7fffffu) == 0x7fffffu ? yfix : __builtin_fmaf(e, y0, y0);
+  }
+  __device__ static bool fits(float a) {
+    const float m = __builtin_fabsf(a);
+    return m == 0.0f || (m >= 0x1p-60f && m <= 0x1p60f);
+  }
+  __device__ float recip() const { return y; }
+  __device__ float div(float a) const {
+    const float q = a * y;
+    const float r = __builtin_fmaf(-b, q, a);
+    return __builtin_copysignf(__builtin_fmaf(r, y, q), a);
+  }
+};
+
+template <>
+struct Divisor<double> {
+  double b;
+  bool ok = true;
+  __device__ explicit Divisor(double b_) : b(b_) {}
+  __device__ static bool fits(double) { return true; }
+  __device__ double recip() const { return 1.0 / b; }
+  __device__ double div(double a) const { return a / b; }
+};
+
 // |q|^2 in the reference's operand order (x, then y, then z).
 template <class real, int DIM>
 __host__ __device__ inline real norm2(real qx, real qy, real qz) {

@@ -718,9 +718,18 @@ __global__ __launch_bounds__(kBlock) void k_vsc(
             const real dist = sqrt(norm2<real, DIM>(qx, qy, qz));
             if (dist <= r) {
               const Vec<real, DIM> vj = vel[j];
-              ax += real(2) * (vj.x - vx) * l0 / dist / dist / dist;
-              ay += real(2) * (vj.y - vy) * l0 / dist / dist / dist;
-              if constexpr (DIM == 3) az += real(2) * (vj.z - vvz) * l0 / dist / dist / dist;
+              const real nx = real(2) * (vj.x - vx) * l0, ny = real(2) * (vj.y - vy) * l0;
+              const real nz = DIM == 3 ? real(2) * (vz(vj) - vvz) * l0 : real(0);
+              const Divisor<real> D(dist);
+              if (D.ok && D.fits(nx) && D.fits(ny) && D.fits(nz)) {
+                ax += D.div(D.div(D.div(nx)));
+                ay += D.div(D.div(D.div(ny)));
+                if constexpr (DIM == 3) az += D.div(D.div(D.div(nz)));
+              } else {
+                ax += nx / dist / dist / dist;
+                ay += ny / dist / dist / dist;
+                if constexpr (DIM == 3) az += nz / dist / dist / dist;
+              }
             }
           });
     }
@@ -739,9 +748,18 @@ __global__ __launch_bounds__(kBlock) void k_vsc(
               const real qx = pj.x - px, qy = pj.y - py, qz = vz(pj) - pz;
               const real dist = sqrt(norm2<real, DIM>(qx, qy, qz));
               if (dist <= r) {
-                ax += real(2) * (real(0) - vx) * l0 / dist / dist / dist;
-                ay += real(2) * (real(0) - vy) * l0 / dist / dist / dist;
-                if constexpr (DIM == 3) az += real(2) * (real(0) - vvz) * l0 / dist / dist / dist;
+                const real nx = real(2) * (real(0) - vx) * l0, ny = real(2) * (real(0) - vy) * l0;
+                const real nz = DIM == 3 ? real(2) * (real(0) - vvz) * l0 : real(0);
+                const Divisor<real> D(dist);
+                if (D.ok && D.fits(nx) && D.fits(ny) && D.fits(nz)) {
+                  ax += D.div(D.div(D.div(nx)));
+                  ay += D.div(D.div(D.div(ny)));
+                  if constexpr (DIM == 3) az += D.div(D.div(D.div(nz)));
+                } else {
+                  ax += nx / dist / dist / dist;
+                  ay += ny / dist / dist / dist;
+                  if constexpr (DIM == 3) az += nz / dist / dist / dist;
+                }
               }
             });
       }
@@ -819,13 +837,21 @@ __global__ void k_wall_pav(real* __restrict__ pav, const real* __restrict__ prs,
 }
 
 // Arithmetic probe: the device's correctly-rounded sqrt and division, used by
-// the tests to show they agree bit for bit with the host's.
+// the tests to show they agree bit for bit with the host's; with `fast`, the
+// quotient comes from Divisor (as the neighbour kernels use it) and s holds
+// its reciprocal.
 template <class real>
-__global__ void k_probe(const real* a, const real* b, real* s, real* q, int n) {
+__global__ void k_probe(const real* a, const real* b, real* s, real* q, int n, int fast) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
-    s[i] = sqrt(a[i]);
-    q[i] = a[i] / b[i];
+    if (fast) {
+      const Divisor<real> D(b[i]);
+      s[i] = D.recip();
+      q[i] = D.ok && D.fits(a[i]) ? D.div(a[i]) : a[i] / b[i];
+    } else {
+      s[i] = sqrt(a[i]);
+      q[i] = a[i] / b[i];
+    }
   }
 }
 

@@ -384,6 +384,21 @@ def compute_edge(params: Params, scene: SceneArrays) -> np.ndarray:
     return e
 
 
+def probe_division(precision, a, b):
+    """(reciprocal, quotient) of the neighbour kernels' reused-divisor division."""
+    lib = load_library()
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    y = np.zeros_like(a)
+    q = np.zeros_like(a)
+    lib.mps_probe_division.restype = C.c_int
+    rc = lib.mps_probe_division(precision, C.c_void_p(a.ctypes.data), C.c_void_p(b.ctypes.data),
+                                C.c_void_p(y.ctypes.data), C.c_void_p(q.ctypes.data), len(a))
+    if rc:
+        raise MPSError(lib.mps_last_error().decode())
+    return y, q
+
+
 def probe_arith(precision, a, b):
     lib = load_library()
     a = np.ascontiguousarray(a, np.float64)

@@ -32,6 +32,37 @@ def test_device_arithmetic_is_correctly_rounded():
         assert np.array_equal(q.astype(dt), aa / bb), f"division differs ({dt})"
 
 
+def test_reused_divisor_division_is_correctly_rounded():
+    """The neighbour kernels' FP32 division (hardware reciprocal refined to
+    RN(1/b), one correction per quotient) equals IEEE a/b bit for bit:
+    the reciprocal for every significand in four binades, quotients on 10^7
+    log-uniform operand pairs, and the fallback outside the fast range."""
+    f32 = np.float32
+    sig = (np.arange(1 << 23, dtype=np.uint32) | np.uint32(127 << 23)).view(f32)  # all of [1, 2)
+    for scale in (2.0 ** -20, 2.0 ** -7, 1.0, 2.0 ** 19):
+        b = (sig * f32(scale)).astype(np.float64)
+        y, _ = ffi.probe_division(ffi.FP32, np.ones_like(b), b)
+        want = f32(1) / b.astype(f32)
+        bad = np.nonzero(y.astype(f32) != want)[0]
+        assert len(bad) == 0, f"RN(1/b) missed for {len(bad)} b in binade {scale}: first b={b[bad[0]]!r}"
+    rng = np.random.default_rng(1)
+    n = 10_000_000
+    a = (np.exp2(rng.uniform(-60, 60, n)) * rng.choice([-1.0, 1.0], n)).astype(f32)
+    b = np.exp2(rng.uniform(-20, 20, n)).astype(f32)
+    a[:4] = [0.0, -0.0, 1.0, -3.0]
+    _, q = ffi.probe_division(ffi.FP32, a.astype(np.float64), b.astype(np.float64))
+    want = a / b
+    bad = np.nonzero(q.astype(f32).view(np.uint32) != want.view(np.uint32))[0]
+    assert len(bad) == 0, f"{len(bad)} quotients differ: a={a[bad[0]]!r} b={b[bad[0]]!r}"
+    # outside the fast range the division falls back to IEEE
+    a2 = np.array([1e-30, 1.0, 1.0, 3.0, 0.0, 1.0, -2.5e30], f32)
+    b2 = np.array([3.0, 0.0, 1e-10, 1e10, 2.0, 2.0 ** 21, 0.7], f32)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        want2 = a2 / b2
+    _, q2 = ffi.probe_division(ffi.FP32, a2.astype(np.float64), b2.astype(np.float64))
+    assert np.array_equal(q2.astype(f32).view(np.uint32), want2.view(np.uint32))
+
+
 CASES = [
     ("small2d-fp64", lambda: configs.small2d("fp64")),
     ("small2d-fp32", lambda: configs.small2d("fp32")),
