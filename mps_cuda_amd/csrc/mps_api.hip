@@ -75,9 +75,11 @@ struct ImplBase {
     allocs.push_back(*p);
     return MPS_OK;
   }
+  // Every array carries mps::kScanPad spare entries: the candidate scans
+  // read up to 7 entries past the end of a range (those results are dropped).
   template <class T>
   int alloc(T** p, size_t count) {
-    return alloc_raw((void**)p, count * sizeof(T));
+    return alloc_raw((void**)p, (count + mps::kScanPad) * sizeof(T));
   }
   void free_all() {
     for (void* p : allocs) (void)hipFree(p);
@@ -204,6 +206,7 @@ struct Impl : ImplBase {
   int *mraw_id = nullptr, *mraw_key = nullptr, *mid = nullptr, *mcs = nullptr, *mcount = nullptr;
   int *near_list = nullptr, *near_scene = nullptr;  // fluid particles with walls in their stencil
   int *nlist = nullptr, *ncount = nullptr;          // ordered fluid neighbours within rp
+  mps::Chunk* chunks = nullptr;                     // per-particle hit records of VscTrm's scan
   uint8_t *wnear = nullptr, *mnear = nullptr;       // any wall / mirror in a bucket's stencil
   // bucket scratch (shared by the fluid and mirror builds)
   int *key = nullptr, *slot = nullptr, *tmp = nullptr, *perm = nullptr, *cnt = nullptr;
@@ -344,7 +347,9 @@ struct Impl : ImplBase {
         (rc = alloc(&mid, mcap)) || (rc = alloc(&mcs, (size_t)tot + 2)) || (rc = alloc(&mcount, 1)) ||
         (rc = alloc(&near_list, std::max(nf, 1))) || (rc = alloc(&near_scene, std::max(nf, 1))) ||
         (rc = alloc(&nlist, (size_t)mps::kNList * std::max(nf, 1))) ||
-        (rc = alloc(&ncount, std::max(nf, 1))) || (rc = alloc(&wnear, std::max(tot, 1))) ||
+        (rc = alloc(&ncount, std::max(nf, 1))) ||
+        (rc = alloc(&chunks, (size_t)mps::kChunkCap * std::max(nf, 1))) ||
+        (rc = alloc(&wnear, std::max(tot, 1))) ||
         (rc = alloc(&mnear, std::max(tot, 1))))
       return rc;
     const size_t scr = std::max(mcap, nf);
@@ -562,15 +567,15 @@ struct Impl : ImplBase {
     const dim3 SB(block), SG(sb_grid);
     if (fuse) {
       TLAUNCH("vsc_up1", (mps::k_vsc<real, DIM, true>), SG, SB, 0, stream, fpos[0], fvel[0],
-              (const int*)fcs, sc(), mpos, mcs, (const uint8_t*)mnear, (V*)nullptr, fpos[1], fvel[1],
-              ftyp[0], fprs[0]);
+              (const int*)fcs, sc(), mpos, mcs, (const uint8_t*)mnear, chunks, nf, (V*)nullptr, fpos[1],
+              fvel[1], ftyp[0], fprs[0]);
       copy_tail(fpos[0], fpos[1], fvel[0], fvel[1], nullptr, nullptr, nullptr);
       std::swap(fpos[0], fpos[1]);
       std::swap(fvel[0], fvel[1]);
     } else {
       TLAUNCH("vsc", (mps::k_vsc<real, DIM, false>), SG, SB, 0, stream, fpos[0], fvel[0],
-              (const int*)fcs, sc(), mpos, mcs, (const uint8_t*)mnear, facc[0], (V*)nullptr, (V*)nullptr,
-              (int8_t*)nullptr, (real*)nullptr);
+              (const int*)fcs, sc(), mpos, mcs, (const uint8_t*)mnear, chunks, nf, facc[0], (V*)nullptr,
+              (V*)nullptr, (int8_t*)nullptr, (real*)nullptr);
     }
   }
 
@@ -944,7 +949,7 @@ struct Impl : ImplBase {
 
   template <class T>
   int aalloc(std::vector<void*>& list, T** p, size_t n) {
-    hipError_t e = hipMalloc((void**)p, std::max<size_t>(n * sizeof(T), 256));
+    hipError_t e = hipMalloc((void**)p, std::max<size_t>((n + mps::kScanPad) * sizeof(T), 256));
     if (e != hipSuccess)
       return set_err(e == hipErrorOutOfMemory ? MPS_E_OOM : MPS_E_HIP,
                      std::string("hipMalloc (adjoint): ") + hipGetErrorString(e));

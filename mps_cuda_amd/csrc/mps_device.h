@@ -152,17 +152,21 @@ template <>
 struct Divisor<float> {
   float b, y;
   bool ok;
+  // The range tests compare bit patterns: for a non-negative float u,
+  // (u - lo) <= (hi - lo) as unsigned integers holds exactly for lo <= u <= hi
+  // (and fails for NaN and infinity).  0x35800000 = 2^-20, 0x49800000 = 2^20,
+  // 0x21800000 = 2^-60, 0x5d800000 = 2^60.  No branches.
   __device__ explicit Divisor(float b_) : b(b_) {
-    ok = b >= 0x1p-20f && b <= 0x1p20f;
+    const unsigned bits = __float_as_uint(b);
+    ok = bits - 0x35800000u <= 0x14000000u;
     const float y0 = __builtin_amdgcn_rcpf(b);
     const float e = __builtin_fmaf(-b, y0, 1.0f);
-    const unsigned bits = __float_as_uint(b);
     const float yfix = __uint_as_float(((253u - ((bits >> 23) & 0xffu)) << 23) | 1u);
     y = (bits & 0x7fffffu) == 0x7fffffu ? yfix : __builtin_fmaf(e, y0, y0);
   }
   __device__ static bool fits(float a) {
-    const float m = __builtin_fabsf(a);
-    return m == 0.0f || (m >= 0x1p-60f && m <= 0x1p60f);
+    const unsigned u = __float_as_uint(a) & 0x7fffffffu;
+    return (u == 0u) | (u - 0x21800000u <= 0x3c000000u);
   }
   __device__ float recip() const { return y; }
   __device__ float div(float a) const {

@@ -659,35 +659,107 @@ __device__ inline bool chk_pcl(real& x, real& y, real& z, real& vx, real& vy, re
 }
 
 // ---------------------------------------------------------------------------
-// Visits, in ascending order, the candidates j of [b, e) that pass test(j)
-// and calls body(j) on each.  The cheap test runs over a chunk of 64
-// candidates first and the hits are kept in a bit mask; the expensive body
-// then runs over the hits only.  Without this, the lanes of a wavefront
-// (each with its own hits) would together take the expensive branch for
-// almost every candidate.  The visiting order is unchanged.
-template <class T, class Body>
-__device__ inline void for_hits(int b, int e, T&& test, Body&& body) {
+// Candidate scans.  A neighbour loop tests the candidates [b, e) of a
+// stencil row with a cheap, conservative distance test and runs its exact
+// body on the hits in ascending order.  scan_chunks runs the test over
+// chunks of 64 candidates, eight at a time so that eight loads are in flight
+// and share one address, and hands each non-empty chunk to emit(base, mask):
+// bit t of mask stands for candidate base + t.  The test reads up to 7
+// entries past e and those bits are dropped, so every array it reads holds
+// kScanPad spare entries after its last one.  test must have no side effects.
+constexpr int kScanPad = 16;
+
+template <class Test, class Emit>
+__device__ inline void scan_chunks(int b, int e, Test&& test, Emit&& emit) {
   for (int base = b; base < e; base += 64) {
     const int n = min(64, e - base);
     unsigned long long mask = 0;
-    for (int t = 0; t < n; t++)
-      if (test(base + t)) mask |= 1ull << t;
-    while (mask) {
-      const int t = __ffsll((long long)mask) - 1;
-      mask &= mask - 1;
-      body(base + t);
+    for (int t = 0; t < n; t += 8) {
+      unsigned bits = 0;
+#pragma unroll
+      for (int u = 0; u < 8; u++) bits |= unsigned(test(base + t + u)) << u;
+      mask |= (unsigned long long)bits << t;
     }
+    if (n < 64) mask &= (1ull << n) - 1ull;
+    if (mask) emit(base, mask);
+  }
+}
+
+// body(base + t) for the set bits t of mask, in ascending order.
+template <class Body>
+__device__ inline void for_bits(int base, unsigned long long mask, Body&& body) {
+  while (mask) {
+    const int t = __ffsll((long long)mask) - 1;
+    mask &= mask - 1;
+    body(base + t);
+  }
+}
+
+// Visits, in ascending order, the candidates j of [b, e) that pass test(j)
+// and calls body(j) on each, one chunk of 64 candidates at a time.
+template <class T, class Body>
+__device__ inline void for_hits(int b, int e, T&& test, Body&& body) {
+  scan_chunks(b, e, test, [&](int base, unsigned long long mask) { for_bits(base, mask, body); });
+}
+
+// Conservative squared distance for the prefilters: fused multiply-adds,
+// so it may differ from norm2 in the last bits; the radii it is compared
+// with carry a 2^-10 margin.
+__device__ inline float near2(float x, float y, float z) {
+  return __builtin_fmaf(z, z, __builtin_fmaf(y, y, x * x));
+}
+__device__ inline double near2(double x, double y, double z) {
+  return __builtin_fma(z, z, __builtin_fma(y, y, x * x));
+}
+
+// A particle's hit record: the chunks of its candidate scan with at least
+// one hit, in scan order.  Hits are base + t for the set bits t of mask;
+// bit 31 of base marks a mirror-set chunk.  Entry k of particle i is kept
+// at buf[k * n_alloc + i], so lanes that read the same k read consecutive
+// records.  A scan that fills kChunkCap entries walks them and starts over.
+struct alignas(16) Chunk {
+  int base, pad;
+  unsigned long long mask;
+};
+constexpr int kChunkCap = 32;
+constexpr int kMirrorChunk = int(0x80000000u);
+
+// Walks chunks [0, count) of particle i and calls body(mirror, j) for every
+// hit in recorded order.  Every stored chunk has a hit, so each iteration
+// gives every lane one hit until its record is done: the loop runs as many
+// times as the lane with the most hits needs, whatever the chunk layout of
+// the other lanes.  The next record is loaded one chunk ahead.
+template <class Body>
+__device__ inline void walk_chunks(const Chunk* buf, size_t n_alloc, int i, int count, Body&& body) {
+  int k = 0, base = 0;
+  unsigned long long m = 0;
+  Chunk nx = {0, 0, 0};
+  if (count > 0) nx = buf[i];
+  for (;;) {
+    if (m == 0) {
+      if (k >= count) break;
+      base = nx.base;
+      m = nx.mask;
+      if (++k < count) nx = buf[(size_t)k * n_alloc + i];
+    }
+    const int t = __ffsll((long long)m) - 1;
+    m &= m - 1;
+    body(base < 0, (base & 0x7fffffff) + t);
   }
 }
 
 // ---------------------------------------------------------------------------
 // d_VscTrm (kernel.cu:1013-1119), optionally fused with d_UpPcl1
 // (kernel.cu:1138-1149).  Scene-uniform workgroups over live particles.
+// Pass 1 scans the fluid rows, then (near walls) the mirror rows, into the
+// particle's hit record; pass 2 walks the record and sums the viscosity
+// terms in that order: fluid neighbours by stencil order, then mirrors,
+// as the reference's two loops do.
 template <class real, int DIM, bool FUSE>
 __global__ __launch_bounds__(kBlock) void k_vsc(
     const Vec<real, DIM>* __restrict__ pos, const Vec<real, DIM>* __restrict__ vel,
     const int* __restrict__ fcs, Scenes<real> sc, const Vec<real, DIM>* __restrict__ mpos,
-    const int* __restrict__ mcs, const uint8_t* __restrict__ mnear,
+    const int* __restrict__ mcs, const uint8_t* __restrict__ mnear, Chunk* chunks, int n_alloc,
     Vec<real, DIM>* __restrict__ acc_out, Vec<real, DIM>* __restrict__ pos_out,
     Vec<real, DIM>* __restrict__ vel_out, int8_t* __restrict__ typ, real* __restrict__ prs) {
   int s, first, end;
@@ -701,68 +773,71 @@ __global__ __launch_bounds__(kBlock) void k_vsc(
   real ax = 0, ay = 0, az = 0;
   const Cell<real, DIM> q = cell_of<real, DIM>(p, c);
   const int zlo = DIM == 3 ? q.iz - 1 : 0, zhi = DIM == 3 ? q.iz + 1 : 0;
-  const real l0 = c.l0, r = c.r;
-  const real r2_hi = c.r2_hi;
+  const real l0 = c.l0, r = c.r, r2_hi = c.r2_hi;
+
+  // One neighbour at pj with velocity u (0 for a mirror): kernel.cu:1056-1064
+  // and 1090-1098, (((2 (u - v)) l0) / dist / dist / dist) per component.
+  auto term = [&](const Vec<real, DIM>& pj, const Vec<real, DIM>& u) {
+    const real qx = pj.x - px, qy = pj.y - py, qz = vz(pj) - pz;
+    const real dist = sqrt(norm2<real, DIM>(qx, qy, qz));
+    if (dist <= r) {
+      const real nx = real(2) * (u.x - vx) * l0, ny = real(2) * (u.y - vy) * l0;
+      const real nz = DIM == 3 ? real(2) * (vz(u) - vvz) * l0 : real(0);
+      const Divisor<real> D(dist);
+      if (D.ok & D.fits(nx) & D.fits(ny) & D.fits(nz)) {
+        ax += D.div(D.div(D.div(nx)));
+        ay += D.div(D.div(D.div(ny)));
+        if constexpr (DIM == 3) az += D.div(D.div(D.div(nz)));
+      } else {
+        ax += nx / dist / dist / dist;
+        ay += ny / dist / dist / dist;
+        if constexpr (DIM == 3) az += nz / dist / dist / dist;
+      }
+    }
+  };
+  int nch = 0;
+  auto walk = [&]() {
+    walk_chunks(chunks, (size_t)n_alloc, i, nch, [&](bool mir, int j) {
+      const Vec<real, DIM>* src = mir ? mpos : pos;
+      const Vec<real, DIM> pj = src[j];
+      Vec<real, DIM> u = mkvec<real, DIM>(0, 0, 0);
+      if (!mir) u = vel[j];
+      term(pj, u);
+    });
+    nch = 0;
+  };
+  auto emit = [&](int base, unsigned long long m) {
+    if (nch == kChunkCap) walk();
+    chunks[(size_t)nch * n_alloc + i] = Chunk{base, 0, m};
+    nch++;
+  };
   for (int jz = zlo; jz <= zhi; jz++)
     for (int jy = q.iy - 1; jy <= q.iy + 1; jy++) {
       const int b = fcs[cell_key(q.ix - 1, jy, jz, c)], e = fcs[cell_key(q.ix + 1, jy, jz, c) + 1];
-      for_hits(
+      scan_chunks(
           b, e,
           [&](int j) {
             const Vec<real, DIM> pj = pos[j];
-            return j != i && norm2<real, DIM>(pj.x - px, pj.y - py, vz(pj) - pz) <= r2_hi;
+            return near2(pj.x - px, pj.y - py, vz(pj) - pz) <= r2_hi;
           },
-          [&](int j) {
-            const Vec<real, DIM> pj = pos[j];
-            const real qx = pj.x - px, qy = pj.y - py, qz = vz(pj) - pz;
-            const real dist = sqrt(norm2<real, DIM>(qx, qy, qz));
-            if (dist <= r) {
-              const Vec<real, DIM> vj = vel[j];
-              const real nx = real(2) * (vj.x - vx) * l0, ny = real(2) * (vj.y - vy) * l0;
-              const real nz = DIM == 3 ? real(2) * (vz(vj) - vvz) * l0 : real(0);
-              const Divisor<real> D(dist);
-              if (D.ok && D.fits(nx) && D.fits(ny) && D.fits(nz)) {
-                ax += D.div(D.div(D.div(nx)));
-                ay += D.div(D.div(D.div(ny)));
-                if constexpr (DIM == 3) az += D.div(D.div(D.div(nz)));
-              } else {
-                ax += nx / dist / dist / dist;
-                ay += ny / dist / dist / dist;
-                if constexpr (DIM == 3) az += nz / dist / dist / dist;
-              }
-            }
+          [&](int base, unsigned long long m) {
+            if (unsigned(i - base) < 64u) m &= ~(1ull << (i - base));  // j != i
+            if (m) emit(base, m);
           });
     }
   if (mnear[cell_key(q.ix, q.iy, q.iz, c)])
     for (int jz = zlo; jz <= zhi; jz++)
       for (int jy = q.iy - 1; jy <= q.iy + 1; jy++) {
         const int b = mcs[cell_key(q.ix - 1, jy, jz, c)], e = mcs[cell_key(q.ix + 1, jy, jz, c) + 1];
-        for_hits(
+        scan_chunks(
             b, e,
             [&](int j) {
               const Vec<real, DIM> pj = mpos[j];
-              return norm2<real, DIM>(pj.x - px, pj.y - py, vz(pj) - pz) <= r2_hi;
+              return near2(pj.x - px, pj.y - py, vz(pj) - pz) <= r2_hi;
             },
-            [&](int j) {
-              const Vec<real, DIM> pj = mpos[j];
-              const real qx = pj.x - px, qy = pj.y - py, qz = vz(pj) - pz;
-              const real dist = sqrt(norm2<real, DIM>(qx, qy, qz));
-              if (dist <= r) {
-                const real nx = real(2) * (real(0) - vx) * l0, ny = real(2) * (real(0) - vy) * l0;
-                const real nz = DIM == 3 ? real(2) * (real(0) - vvz) * l0 : real(0);
-                const Divisor<real> D(dist);
-                if (D.ok && D.fits(nx) && D.fits(ny) && D.fits(nz)) {
-                  ax += D.div(D.div(D.div(nx)));
-                  ay += D.div(D.div(D.div(ny)));
-                  if constexpr (DIM == 3) az += D.div(D.div(D.div(nz)));
-                } else {
-                  ax += nx / dist / dist / dist;
-                  ay += ny / dist / dist / dist;
-                  if constexpr (DIM == 3) az += nz / dist / dist / dist;
-                }
-              }
-            });
+            [&](int base, unsigned long long m) { emit(base | kMirrorChunk, m); });
       }
+  walk();
   const real accx = c.n0knm * ax + c.g[0];
   const real accy = c.n0knm * ay + c.g[1];
   const real accz = DIM == 3 ? c.n0knm * az + c.g[2] : real(0);

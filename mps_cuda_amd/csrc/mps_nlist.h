@@ -35,15 +35,18 @@ __global__ __launch_bounds__(kBlock) void k_nlist_build(const Vec<real, DIM>* __
   for (int jz = zlo; jz <= zhi; jz++)
     for (int jy = q.iy - 1; jy <= q.iy + 1; jy++) {
       const int b = fcs[cell_key(q.ix - 1, jy, jz, c)], e = fcs[cell_key(q.ix + 1, jy, jz, c) + 1];
-      for_hits(
+      scan_chunks(
           b, e,
           [&](int j) {
             const Vec<real, DIM> pj = pos[j];
-            return j != i && norm2<real, DIM>(pj.x - px, pj.y - py, vz(pj) - pz) <= lim;
+            return near2(pj.x - px, pj.y - py, vz(pj) - pz) <= lim;
           },
-          [&](int j) {
-            if (n < kNList) list[(size_t)n * n_alloc + i] = j;
-            n++;
+          [&](int base, unsigned long long m) {
+            if (unsigned(i - base) < 64u) m &= ~(1ull << (i - base));  // j != i
+            for_bits(base, m, [&](int j) {
+              if (n < kNList) list[(size_t)n * n_alloc + i] = j;
+              n++;
+            });
           });
     }
   count[i] = n <= kNList ? n : -1;
