@@ -50,6 +50,7 @@ def algorithmic_bytes(kernel: str, nf: int, nw: int, m: int) -> float:
         "vsc_up1": 49 * nf + 12 * m,       # pos, vel in; pos, vel out; typ
         "col": 37 * nf + 13 * nw + 12 * m,
         "prs": 17 * nf + 12 * m,
+        "col_prs": 42 * nf + 13 * nw + 12 * m,  # col + prs, positions and mirrors read once
         "grad_up2": 61 * nf + 16 * m,
         "mirror_gen": 13 * nf + 26 * nw + 29 * m,
     }
@@ -59,6 +60,7 @@ def algorithmic_bytes(kernel: str, nf: int, nw: int, m: int) -> float:
 KERNEL_SYMBOL = {  # timing name -> kernel symbol in the rocprofv3 summaries
     "vsc_up1": "mps::k_vsc<float, 3, true>",
     "col": "mps::k_col_list<float, 3, false>",
+    "col_prs": "mps::k_colprs_list<float, 3>",
     "prs": "mps::k_prs_list<float, 3>",
     "grad_up2": "mps::k_grad_list<float, 3, true>",
     "mirror_gen": "mps::k_mirror_wave<float, 3>",
@@ -250,7 +252,7 @@ def plain_leg(args) -> dict:
     nlive = int((h.state()["typ"] == ffi.FLD).sum())
     bytes_ = algorithmic_bytes(dom, nlive, nw, m)
     achieved = bytes_ / (avg_ms * 1e-3) / 1e9
-    passes = {"vsc_up1": 1, "prs": 1, "grad_up2": 2, "col": 1}.get(dom, 1)
+    passes = {"vsc_up1": 1, "prs": 1, "grad_up2": 2, "col": 1, "col_prs": 1}.get(dom, 1)
     flops = FLOP_PER_CANDIDATE * candidates_per_pass(h) * passes
     tflops = flops / (avg_ms * 1e-3) / 1e12
     traffic = measured_traffic(dom)

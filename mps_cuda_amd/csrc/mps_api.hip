@@ -600,6 +600,16 @@ struct Impl : ImplBase {
     std::swap(fvel[0], fvel[1]);
   }
 
+  // ChkCol + UpChkCol and MkPrs in one launch (the step's fused path):
+  // new velocities into fvel[1], swapped in; pressure and number density.
+  void run_colprs() {
+    TLAUNCH("col_prs", (mps::k_colprs_list<real, DIM>), dim3(sb_grid), dim3(block), 0, stream, fpos[0],
+            fvel[0], (const int*)fcs, nf, nlist, ncount, wpos, wvel, wcs, mpos, mcs, (const uint8_t*)wnear,
+            (const uint8_t*)mnear, sc(), fvel[1], fprs[0], fnd[0]);
+    copy_tail(fvel[0], fvel[1], nullptr, nullptr, nullptr, nullptr, nullptr);
+    std::swap(fvel[0], fvel[1]);
+  }
+
   void run_prs() {
     TLAUNCH("prs", (mps::k_prs_list<real, DIM>), dim3(sb_grid), dim3(block), 0, stream, fpos[0],
             (const int*)fcs, nf, nlist, ncount, mpos, mcs, (const uint8_t*)mnear, sc(), fprs[0], fnd[0]);
@@ -640,8 +650,7 @@ struct Impl : ImplBase {
       bucket();
       if (nf > 0) {
         run_nlist();
-        run_col(false);
-        run_prs();
+        run_colprs();
       }
       mirror();
       if (nf > 0) run_grad(true);

@@ -72,6 +72,197 @@ __device__ inline void for_fluid_near(int i, int n_alloc, const int* __restrict_
     }
 }
 
+// Calls f(j, pos[j]) for the fluid candidates of particle i within rp (with
+// margin) in visiting order.  From the list, eight entries and then their
+// positions are loaded together before f runs on them in order; a particle
+// whose list overflowed scans its stencil rows instead.  Every consumer's
+// own radius test (d2 < rp2, sqrt(d2) < rp, d2 < rlim2) lies inside rp2_hi,
+// so both sources give it the same candidates.
+// With aux != nullptr, f(j, pos[j], aux[j]) is called and aux is loaded
+// along with the positions.
+template <class real, int DIM, class F>
+__device__ inline void for_near(int i, real px, real py, real pz, size_t n_alloc,
+                                const int* __restrict__ list, int n,
+                                const Vec<real, DIM>* __restrict__ pos, const int* __restrict__ fcs,
+                                const Cell<real, DIM>& q, const Cst<real>& c, F&& f,
+                                const real* __restrict__ aux = nullptr) {
+  if (n >= 0) {
+    for (int k = 0; k < n; k += 8) {
+      int j[8];
+      Vec<real, DIM> pj[8];
+      real a[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) j[u] = list[(size_t)min(k + u, n - 1) * n_alloc + i];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        pj[u] = pos[j[u]];
+        a[u] = aux ? aux[j[u]] : real(0);
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u++)
+        if (k + u < n) f(j[u], pj[u], a[u]);
+    }
+    return;
+  }
+  const int zlo = DIM == 3 ? q.iz - 1 : 0, zhi = DIM == 3 ? q.iz + 1 : 0;
+  const real lim = c.rp2_hi;
+  for (int jz = zlo; jz <= zhi; jz++)
+    for (int jy = q.iy - 1; jy <= q.iy + 1; jy++) {
+      const int b = fcs[cell_key(q.ix - 1, jy, jz, c)], e = fcs[cell_key(q.ix + 1, jy, jz, c) + 1];
+      for_hits(
+          b, e,
+          [&](int j) {
+            const Vec<real, DIM> pj = pos[j];
+            return near2(pj.x - px, pj.y - py, vz(pj) - pz) <= lim;
+          },
+          [&](int j) {
+            if (j != i) f(j, pos[j], aux ? aux[j] : real(0));
+          });
+    }
+}
+
+// Mirror candidates of the stencil within rp (with margin), in order.
+template <class real, int DIM, class F>
+__device__ inline void for_mirrors_near(real px, real py, real pz, const Vec<real, DIM>* __restrict__ mpos,
+                                        const int* __restrict__ mcs, const Cell<real, DIM>& q,
+                                        const Cst<real>& c, F&& f) {
+  const int zlo = DIM == 3 ? q.iz - 1 : 0, zhi = DIM == 3 ? q.iz + 1 : 0;
+  const real lim = c.rp2_hi;
+  for (int jz = zlo; jz <= zhi; jz++)
+    for (int jy = q.iy - 1; jy <= q.iy + 1; jy++) {
+      const int b = mcs[cell_key(q.ix - 1, jy, jz, c)], e = mcs[cell_key(q.ix + 1, jy, jz, c) + 1];
+      for_hits(
+          b, e,
+          [&](int j) {
+            const Vec<real, DIM> pj = mpos[j];
+            return near2(pj.x - px, pj.y - py, vz(pj) - pz) <= lim;
+          },
+          [&](int j) { f(j, mpos[j]); });
+    }
+}
+
+// ---------------------------------------------------------------------------
+// d_ChkCol + d_UpChkCol (kernel.cu:1168-1287) and d_MkPrs (kernel.cu:1306-1411)
+// in one walk of the list: both read the positions after UpPcl1 and the
+// mirrors generated before it, and neither reads what the other writes.
+// Each keeps its own order: MkPrs sums fluid neighbours in list order, then
+// mirrors; ChkCol applies contacts per stencil bucket (fluid, then that
+// bucket's walls), then mirrors.  Contacts (d2 < (0.5 l0)^2 for fluid and
+// mirrors, (0.45 l0)^2 for walls) are rare: a particle without walls in its
+// stencil applies fluid contacts during the walk; one with walls applies its
+// wall contacts after the walk when it had no fluid contact, and otherwise
+// replays the reference's bucket-by-bucket order.
+template <class real, int DIM>
+__global__ __launch_bounds__(kBlock) void k_colprs_list(
+    const Vec<real, DIM>* __restrict__ pos, const Vec<real, DIM>* __restrict__ vel,
+    const int* __restrict__ fcs, int n_alloc, const int* __restrict__ list,
+    const int* __restrict__ count, const Vec<real, DIM>* __restrict__ wpos,
+    const Vec<real, DIM>* __restrict__ wvel, const int* __restrict__ wcs,
+    const Vec<real, DIM>* __restrict__ mpos, const int* __restrict__ mcs,
+    const uint8_t* __restrict__ wnear, const uint8_t* __restrict__ mnear, Scenes<real> sc,
+    Vec<real, DIM>* __restrict__ vel_out, real* __restrict__ prs, real* __restrict__ nd) {
+  int s, first, end;
+  Cst<real> c;
+  if (!scene_block(sc, fcs, s, c, first, end)) return;
+  const int i = first + threadIdx.x;
+  if (i >= end) return;
+  const Vec<real, DIM> p = pos[i], v = vel[i];
+  const real px = p.x, py = p.y, pz = vz(p);
+  const real vx = v.x, vy = v.y, vvz = vz(v);
+  const real rp = c.rp, lim = c.rp2_hi, rlim2 = c.rlim2;
+  real v2x = vx, v2y = vy, v2z = vvz;
+  real ni = 0;
+  const real mi = c.mass, mj = c.mass;
+  auto apply = [&](real qx, real qy, real qz, real dist2, const Vec<real, DIM>& u) {
+    real fDT = dot3<real, DIM>(vx - u.x, vy - u.y, vvz - vz(u), qx, qy, qz);
+    if (fDT > real(0)) {
+      fDT *= c.col * mj / (mi + mj) / dist2;
+      v2x -= qx * fDT;
+      v2y -= qy * fDT;
+      if constexpr (DIM == 3) v2z -= qz * fDT;
+    }
+  };
+  auto contact = [&](const Vec<real, DIM>& pj, auto&& vel_of, real lim2) {
+    const real qx = pj.x - px, qy = pj.y - py, qz = vz(pj) - pz;
+    const real dist2 = norm2<real, DIM>(qx, qy, qz);
+    if (dist2 < lim2) apply(qx, qy, qz, dist2, vel_of());
+  };
+  const Cell<real, DIM> q = cell_of<real, DIM>(p, c);
+  const int hk = cell_key(q.ix, q.iy, q.iz, c);
+  const bool walls = wnear[hk];
+  const int n = count[i];
+  int fluid_contacts = 0;
+  for_near<real, DIM>(i, px, py, pz, (size_t)n_alloc, list, n, pos, fcs, q, c,
+                      [&](int j, const Vec<real, DIM>& pj, real) {
+                        const real qx = pj.x - px, qy = pj.y - py, qz = vz(pj) - pz;
+                        const real d2 = norm2<real, DIM>(qx, qy, qz);
+                        if (d2 <= lim) {
+                          const real dist = sqrt(d2);
+                          if (dist < rp) ni += (rp / dist) - (dist / rp);
+                        }
+                        if (d2 < rlim2) {
+                          if (!walls) apply(qx, qy, qz, d2, vel[j]);
+                          fluid_contacts++;
+                        }
+                      });
+  if (walls) {
+    const int zlo = DIM == 3 ? q.iz - 1 : 0, zhi = DIM == 3 ? q.iz + 1 : 0;
+    if (fluid_contacts == 0) {
+      const real wl = c.rlim_wall2;
+      for (int jz = zlo; jz <= zhi; jz++)
+        for (int jy = q.iy - 1; jy <= q.iy + 1; jy++) {
+          const int b = wcs[cell_key(q.ix - 1, jy, jz, c)], e = wcs[cell_key(q.ix + 1, jy, jz, c) + 1];
+          for_hits(
+              b, e,
+              [&](int j) {
+                const Vec<real, DIM> pj = wpos[j];
+                return norm2<real, DIM>(pj.x - px, pj.y - py, vz(pj) - pz) < wl;
+              },
+              [&](int j) { contact(wpos[j], [&] { return wvel[j]; }, wl); });
+        }
+    } else {
+      int k = 0;
+      for (int jz = zlo; jz <= zhi; jz++)
+        for (int jy = q.iy - 1; jy <= q.iy + 1; jy++)
+          for (int jx = q.ix - 1; jx <= q.ix + 1; jx++) {
+            const int ck = cell_key(jx, jy, jz, c);
+            const int fe = fcs[ck + 1];
+            if (n >= 0) {
+              for (; k < n; k++) {
+                const int j = list[(size_t)k * n_alloc + i];
+                if (j >= fe) break;
+                contact(pos[j], [&] { return vel[j]; }, rlim2);
+              }
+            } else {
+              for (int j = fcs[ck]; j < fe; j++)
+                if (j != i) contact(pos[j], [&] { return vel[j]; }, rlim2);
+            }
+            for (int j = wcs[ck]; j < wcs[ck + 1]; j++)
+              contact(wpos[j], [&] { return wvel[j]; }, c.rlim_wall2);
+          }
+    }
+  }
+  if (mnear[hk])
+    for_mirrors_near<real, DIM>(px, py, pz, mpos, mcs, q, c, [&](int, const Vec<real, DIM>& pj) {
+      const real qx = pj.x - px, qy = pj.y - py, qz = vz(pj) - pz;
+      const real d2 = norm2<real, DIM>(qx, qy, qz);
+      if (d2 <= lim) {
+        const real dist = sqrt(d2);
+        if (dist < rp) ni += (rp / dist) - (dist / rp);
+      }
+      if (d2 < rlim2) apply(qx, qy, qz, d2, mkvec<real, DIM>(0, 0, 0));
+    });
+  vel_out[i] = mkvec<real, DIM>(v2x, v2y, v2z);
+  real pressure = real(ni > c.n0_grad) * (ni - c.n0_grad) * c.prs_coef * c.mass;
+  if (pressure <= c.pmax && pressure >= real(0)) {
+  } else {
+    if (pressure > c.pmax) pressure = c.pmax;
+    else pressure = real(0);
+  }
+  prs[i] = pressure;
+  nd[i] = ni;
+}
+
 // ---------------------------------------------------------------------------
 // d_ChkCol + d_UpChkCol (kernel.cu:1168-1287) over the list.  In each
 // stencil bucket the fluid entries (consecutive in the list) come before
@@ -214,13 +405,14 @@ __global__ __launch_bounds__(kBlock) void k_grad_list(
   real Prs_min = prs_i;
   real ax = 0, ay = 0, az = 0;
   const Cell<real, DIM> q = cell_of<real, DIM>(p, c);
-  for_fluid_near<real, DIM>(i, n_alloc, list, count, fcs, q, c, [&](int j) {
-    const Vec<real, DIM> pj = pos[j];
-    if (norm2<real, DIM>(pj.x - px, pj.y - py, vz(pj) - pz) < rp2) {
-      const real pr = prs[j];
-      if (pr < Prs_min) Prs_min = pr;
-    }
-  });
+  const int n = count[i];
+  // P_min over fluid neighbours with d2 < rp2 (kernel.cu:1686-1700)
+  for_near<real, DIM>(
+      i, px, py, pz, (size_t)n_alloc, list, n, pos, fcs, q, c,
+      [&](int, const Vec<real, DIM>& pj, real pr) {
+        if (norm2<real, DIM>(pj.x - px, pj.y - py, vz(pj) - pz) < rp2 && pr < Prs_min) Prs_min = pr;
+      },
+      prs);
   auto add = [&](const Vec<real, DIM>& pj, real pr) {
     const real qx = pj.x - px, qy = pj.y - py, qz = vz(pj) - pz;
     const real dist2 = norm2<real, DIM>(qx, qy, qz);
@@ -232,14 +424,12 @@ __global__ __launch_bounds__(kBlock) void k_grad_list(
     ay += d * w * qy;
     if constexpr (DIM == 3) az += d * w * qz;
   };
-  for_fluid_near<real, DIM>(i, n_alloc, list, count, fcs, q, c, [&](int j) { add(pos[j], prs[j]); });
-  const int zlo = DIM == 3 ? q.iz - 1 : 0, zhi = DIM == 3 ? q.iz + 1 : 0;
+  for_near<real, DIM>(
+      i, px, py, pz, (size_t)n_alloc, list, n, pos, fcs, q, c,
+      [&](int, const Vec<real, DIM>& pj, real pr) { add(pj, pr); }, prs);
   if (mnear[cell_key(q.ix, q.iy, q.iz, c)])
-  for (int jz = zlo; jz <= zhi; jz++)
-    for (int jy = q.iy - 1; jy <= q.iy + 1; jy++) {
-      const int b = mcs[cell_key(q.ix - 1, jy, jz, c)], e = mcs[cell_key(q.ix + 1, jy, jz, c) + 1];
-      for (int j = b; j < e; j++) add(mpos[j], mprs[j]);
-    }
+    for_mirrors_near<real, DIM>(px, py, pz, mpos, mcs, q, c,
+                                [&](int j, const Vec<real, DIM>& pj) { add(pj, mprs[j]); });
   const real accx = c.invro * ax * c.grad_a;
   const real accy = c.invro * ay * c.grad_a;
   const real accz = DIM == 3 ? c.invro * az * c.grad_a : real(0);
