@@ -38,6 +38,60 @@ __global__ void k_fill(T* a, int n, T v) {
 }
 
 // ---------------------------------------------------------------------------
+// Candidate scans.  A neighbour loop tests the candidates [b, e) of a
+// stencil row with a cheap, conservative distance test and runs its exact
+// body on the hits in ascending order.  scan_chunks runs the test over
+// chunks of 64 candidates, eight at a time so that eight loads are in flight
+// and share one address, and hands each non-empty chunk to emit(base, mask):
+// bit t of mask stands for candidate base + t.  The test reads up to 7
+// entries past e and those bits are dropped, so every array it reads holds
+// kScanPad spare entries after its last one.  test must have no side effects.
+constexpr int kScanPad = 16;
+
+template <class Test, class Emit>
+__device__ inline void scan_chunks(int b, int e, Test&& test, Emit&& emit) {
+  for (int base = b; base < e; base += 64) {
+    const int n = min(64, e - base);
+    unsigned long long mask = 0;
+    for (int t = 0; t < n; t += 8) {
+      unsigned bits = 0;
+#pragma unroll
+      for (int u = 0; u < 8; u++) bits |= unsigned(test(base + t + u)) << u;
+      mask |= (unsigned long long)bits << t;
+    }
+    if (n < 64) mask &= (1ull << n) - 1ull;
+    if (mask) emit(base, mask);
+  }
+}
+
+// body(base + t) for the set bits t of mask, in ascending order.
+template <class Body>
+__device__ inline void for_bits(int base, unsigned long long mask, Body&& body) {
+  while (mask) {
+    const int t = __ffsll((long long)mask) - 1;
+    mask &= mask - 1;
+    body(base + t);
+  }
+}
+
+// Visits, in ascending order, the candidates j of [b, e) that pass test(j)
+// and calls body(j) on each, one chunk of 64 candidates at a time.
+template <class T, class Body>
+__device__ inline void for_hits(int b, int e, T&& test, Body&& body) {
+  scan_chunks(b, e, test, [&](int base, unsigned long long mask) { for_bits(base, mask, body); });
+}
+
+// Conservative squared distance for the prefilters: fused multiply-adds,
+// so it may differ from norm2 in the last bits; the radii it is compared
+// with carry a 2^-10 margin.
+__device__ inline float near2(float x, float y, float z) {
+  return __builtin_fmaf(z, z, __builtin_fmaf(y, y, x * x));
+}
+__device__ inline double near2(double x, double y, double z) {
+  return __builtin_fma(z, z, __builtin_fma(y, y, x * x));
+}
+
+// ---------------------------------------------------------------------------
 // Exclusive scan of int counts (cell occupancy -> cell_start).
 // Tile = kBlock threads x kScanIPT items.
 constexpr int kScanIPT = 8;
@@ -398,6 +452,8 @@ __device__ inline int mirror_key(real mx, real my, real mz, const Cst<real>& c) 
 
 constexpr int kInf = 0x7fffffff;
 
+// One LDS atomic per listed particle and one global atomic per workgroup;
+// the list order is not observed (mirrors are sorted by (cell, id) later).
 template <class real, int DIM>
 __global__ __launch_bounds__(kBlock) void k_mirror_near(const Vec<real, DIM>* __restrict__ fpos,
                                                        const int* __restrict__ fcs,
@@ -406,18 +462,44 @@ __global__ __launch_bounds__(kBlock) void k_mirror_near(const Vec<real, DIM>* __
                                                        int* __restrict__ near_list,
                                                        int* __restrict__ near_scene,
                                                        int* __restrict__ near_count) {
+  __shared__ int s_cnt, s_base;
   int s, first, end;
   Cst<real> c;
-  if (!scene_block(sc, fcs, s, c, first, end)) return;
+  if (!scene_block(sc, fcs, s, c, first, end)) return;  // whole workgroup
+  if (threadIdx.x == 0) s_cnt = 0;
+  __syncthreads();
   const int i = first + threadIdx.x;
-  if (i >= end) return;
-  const Cell<real, DIM> q = cell_of<real, DIM>(fpos[i], c);
-  if (wnear[cell_key(q.ix, q.iy, q.iz, c)]) {
-    const int t = wave_alloc(near_count);
-    near_list[t] = i;
-    near_scene[t] = s;
+  bool near = false;
+  if (i < end) {
+    const Cell<real, DIM> q = cell_of<real, DIM>(fpos[i], c);
+    near = wnear[cell_key(q.ix, q.iy, q.iz, c)] != 0;
+  }
+  int o = 0;
+  if (near) o = atomicAdd(&s_cnt, 1);
+  __syncthreads();
+  if (threadIdx.x == 0) s_base = s_cnt > 0 ? atomicAdd(near_count, s_cnt) : 0;
+  __syncthreads();
+  if (near) {
+    near_list[s_base + o] = i;
+    near_scene[s_base + o] = s;
   }
 }
+
+// The stencil rows of cell q as [b[r], e[r]) in the wall set, r in scan
+// order (z, then y); NR = 9 in 3-D, 3 in 2-D.  All bounds load together.
+template <class real, int DIM>
+struct Rows {
+  static constexpr int NR = DIM == 3 ? 9 : 3;
+  int b[NR], e[NR];
+  __device__ Rows(const int* __restrict__ cs, const Cell<real, DIM>& q, const Cst<real>& c) {
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+      const int jz = DIM == 3 ? q.iz - 1 + r / 3 : 0, jy = q.iy - 1 + r % 3;
+      b[r] = cs[cell_key(q.ix - 1, jy, jz, c)];
+      e[r] = cs[cell_key(q.ix + 1, jy, jz, c) + 1];
+    }
+  }
+};
 
 template <class real>
 struct BestWall {
@@ -426,31 +508,43 @@ struct BestWall {
 };
 
 // Wave-wide nearest wall within r of m (first met wins ties); j = -1 if none.
+// Lane l takes candidates b + l, b + l + 64, ... of every row; the first
+// candidate of each row is loaded for all rows at once.  Every lane keeps
+// the lexicographic (dist, index) minimum of its candidates, so its visiting
+// order does not matter, and the wave reduction gives the overall one: the
+// minimum distance met first in scan order (ascending wall index).
 template <class real, int DIM>
 __device__ inline BestWall<real> wave_nearest_wall(real mx, real my, real mz,
                                                    const Vec<real, DIM>* __restrict__ wpos,
                                                    const int* __restrict__ wcs, const Cst<real>& c,
                                                    int lane) {
   const Cell<real, DIM> q = cell_of<real, DIM>(mkvec<real, DIM>(mx, my, mz), c);
+  const Rows<real, DIM> R(wcs, q, c);
+  constexpr int NR = Rows<real, DIM>::NR;
   real best = real(10) * c.r;
   real lim = c.r2_hi;
   int bj = -1;
-  const int zlo = DIM == 3 ? q.iz - 1 : 0, zhi = DIM == 3 ? q.iz + 1 : 0;
-  for (int jz = zlo; jz <= zhi; jz++)
-    for (int jy = q.iy - 1; jy <= q.iy + 1; jy++) {
-      const int b = wcs[cell_key(q.ix - 1, jy, jz, c)], e = wcs[cell_key(q.ix + 1, jy, jz, c) + 1];
-      for (int j = b + lane; j < e; j += 64) {
-        const Vec<real, DIM> w = wpos[j];
-        const real d2 = norm2<real, DIM>(mx - w.x, my - w.y, mz - vz(w));
-        if (!(d2 <= lim)) continue;
-        const real dist = sqrt(d2);
-        if (dist <= c.r && dist < best) {
-          best = dist;
-          bj = j;
-          lim = d2;
-        }
-      }
+  auto visit = [&](int j, const Vec<real, DIM>& w) {
+    const real d2 = norm2<real, DIM>(mx - w.x, my - w.y, mz - vz(w));
+    if (!(d2 <= lim)) return;
+    const real dist = sqrt(d2);
+    if (dist <= c.r && (dist < best || (dist == best && j < bj))) {
+      best = dist;
+      bj = j;
+      lim = d2;
     }
+  };
+  Vec<real, DIM> w[NR];
+#pragma unroll
+  for (int r = 0; r < NR; r++) {
+    const int j = R.b[r] + lane;
+    w[r] = wpos[j < R.e[r] ? j : 0];
+  }
+#pragma unroll
+  for (int r = 0; r < NR; r++)
+    if (R.b[r] + lane < R.e[r]) visit(R.b[r] + lane, w[r]);
+  for (int r = 0; r < NR; r++)  // rows of more than 64 candidates
+    for (int j = R.b[r] + 64 + lane; j < R.e[r]; j += 64) visit(j, wpos[j]);
   // lexicographic (dist, j) minimum over the wave; lanes without a hit hold j = -1
   for (int off = 32; off >= 1; off >>= 1) {
     const real od = __shfl_xor(best, off);
@@ -500,31 +594,40 @@ __global__ __launch_bounds__(64) void k_mirror_wave(
     if (lane < kNumMRR) s_key[lane] = -1;
     __syncthreads();
     const real r2_2 = real(1) * c.r2;
-    const int zlo = DIM == 3 ? q.iz - 1 : 0, zhi = DIM == 3 ? q.iz + 1 : 0;
-    for (int jz = zlo; jz <= zhi; jz++)
-      for (int jy = q.iy - 1; jy <= q.iy + 1; jy++) {
-        const int b = wcs[cell_key(q.ix - 1, jy, jz, c)], e = wcs[cell_key(q.ix + 1, jy, jz, c) + 1];
-        for (int j = b + lane; j < e; j += 64) {
-          const Vec<real, DIM> w = wpos[j];
-          if (norm2<real, DIM>(px - w.x, py - w.y, pz - vz(w)) <= r2_2) {
-            const int cl = wcls[j];
-            atomicMin(&first_reg[cl & 63], j);
-            if (cl & 0x80) atomicMin(&first_edge[cl & 63], j);
-          }
+    {
+      const Rows<real, DIM> R(wcs, q, c);
+      constexpr int NR = Rows<real, DIM>::NR;
+      auto visit = [&](int j, const Vec<real, DIM>& w, int cl) {
+        if (norm2<real, DIM>(px - w.x, py - w.y, pz - vz(w)) <= r2_2) {
+          atomicMin(&first_reg[cl & 63], j);
+          if (cl & 0x80) atomicMin(&first_edge[cl & 63], j);
         }
+      };
+      Vec<real, DIM> w[NR];
+      int cl[NR];
+#pragma unroll
+      for (int r = 0; r < NR; r++) {
+        const int j = R.b[r] + lane, jj = j < R.e[r] ? j : 0;
+        w[r] = wpos[jj];
+        cl[r] = wcls[jj];
       }
-    __syncthreads();
-    // registry order: rank of each class's first index
-    const int fr = first_reg[lane], fe = first_edge[lane];
-    int rr = 0, re_ = 0;
-    for (int k = 0; k < 64; k++) {
-      rr += first_reg[k] < fr;
-      re_ += first_edge[k] < fe;
+#pragma unroll
+      for (int r = 0; r < NR; r++)
+        if (R.b[r] + lane < R.e[r]) visit(R.b[r] + lane, w[r], cl[r]);
+      for (int r = 0; r < NR; r++)  // rows of more than 64 candidates
+        for (int j = R.b[r] + 64 + lane; j < R.e[r]; j += 64) visit(j, wpos[j], wcls[j]);
     }
+    __syncthreads();
+    // registry order: rank of each class's first index among the classes hit
+    const int fr = first_reg[lane], fe = first_edge[lane];
+    const unsigned long long hit = __ballot(fr != kInf), hite = __ballot(fe != kInf);
+    int rr = 0, re_ = 0;
+    for (unsigned long long m = hit; m; m &= m - 1) rr += first_reg[__ffsll((long long)m) - 1] < fr;
+    for (unsigned long long m = hite; m; m &= m - 1) re_ += first_edge[__ffsll((long long)m) - 1] < fe;
     if (fr != kInf && rr < kNumMRR) sorted_reg[rr] = fr;
     if (fe != kInf && re_ < kNumMRR) sorted_edge[re_] = fe;
-    const int nreg = min(__popcll(__ballot(fr != kInf)), kNumMRR);
-    const int ne = min(__popcll(__ballot(fe != kInf)), kNumMRR);
+    const int nreg = min(__popcll(hit), kNumMRR);
+    const int ne = min(__popcll(hite), kNumMRR);
     __syncthreads();
     const int base_id = fid[i] * kNumMRR;
     const real prs_i = fprs[i];
@@ -656,60 +759,6 @@ __device__ inline bool chk_pcl(real& x, real& y, real& z, real& vx, real& vy, re
   }
   x = y = z = vx = vy = vz_ = real(0);
   return false;
-}
-
-// ---------------------------------------------------------------------------
-// Candidate scans.  A neighbour loop tests the candidates [b, e) of a
-// stencil row with a cheap, conservative distance test and runs its exact
-// body on the hits in ascending order.  scan_chunks runs the test over
-// chunks of 64 candidates, eight at a time so that eight loads are in flight
-// and share one address, and hands each non-empty chunk to emit(base, mask):
-// bit t of mask stands for candidate base + t.  The test reads up to 7
-// entries past e and those bits are dropped, so every array it reads holds
-// kScanPad spare entries after its last one.  test must have no side effects.
-constexpr int kScanPad = 16;
-
-template <class Test, class Emit>
-__device__ inline void scan_chunks(int b, int e, Test&& test, Emit&& emit) {
-  for (int base = b; base < e; base += 64) {
-    const int n = min(64, e - base);
-    unsigned long long mask = 0;
-    for (int t = 0; t < n; t += 8) {
-      unsigned bits = 0;
-#pragma unroll
-      for (int u = 0; u < 8; u++) bits |= unsigned(test(base + t + u)) << u;
-      mask |= (unsigned long long)bits << t;
-    }
-    if (n < 64) mask &= (1ull << n) - 1ull;
-    if (mask) emit(base, mask);
-  }
-}
-
-// body(base + t) for the set bits t of mask, in ascending order.
-template <class Body>
-__device__ inline void for_bits(int base, unsigned long long mask, Body&& body) {
-  while (mask) {
-    const int t = __ffsll((long long)mask) - 1;
-    mask &= mask - 1;
-    body(base + t);
-  }
-}
-
-// Visits, in ascending order, the candidates j of [b, e) that pass test(j)
-// and calls body(j) on each, one chunk of 64 candidates at a time.
-template <class T, class Body>
-__device__ inline void for_hits(int b, int e, T&& test, Body&& body) {
-  scan_chunks(b, e, test, [&](int base, unsigned long long mask) { for_bits(base, mask, body); });
-}
-
-// Conservative squared distance for the prefilters: fused multiply-adds,
-// so it may differ from norm2 in the last bits; the radii it is compared
-// with carry a 2^-10 margin.
-__device__ inline float near2(float x, float y, float z) {
-  return __builtin_fmaf(z, z, __builtin_fmaf(y, y, x * x));
-}
-__device__ inline double near2(double x, double y, double z) {
-  return __builtin_fma(z, z, __builtin_fma(y, y, x * x));
 }
 
 // A particle's hit record: the chunks of its candidate scan with at least
