@@ -209,7 +209,7 @@ struct Impl : ImplBase {
   mps::Chunk* chunks = nullptr;                     // per-particle hit records of VscTrm's scan
   uint8_t *wnear = nullptr, *mnear = nullptr;       // any wall / mirror in a bucket's stencil
   // bucket scratch (shared by the fluid and mirror builds)
-  int *key = nullptr, *slot = nullptr, *tmp = nullptr, *perm = nullptr, *cnt = nullptr;
+  int *key = nullptr, *slot = nullptr, *tmp = nullptr, *tmp_id = nullptr, *perm = nullptr, *cnt = nullptr;
   // scan levels: parts[l] holds block totals, parts_ex[l] their exclusive scan
   std::vector<int*> parts, parts_ex;
   // captured step graph
@@ -354,6 +354,7 @@ struct Impl : ImplBase {
       return rc;
     const size_t scr = std::max(mcap, nf);
     if ((rc = alloc(&key, scr)) || (rc = alloc(&slot, scr)) || (rc = alloc(&tmp, scr)) ||
+        (rc = alloc(&tmp_id, scr)) ||
         (rc = alloc(&perm, scr)) || (rc = alloc(&cnt, (size_t)tot + 1)))
       return rc;
     for (long long n = tot + 1;;) {  // scan levels for n = tot + 1
@@ -498,9 +499,9 @@ struct Impl : ImplBase {
     tick("bucket_scan");
     scan(cnt, tot + 1, fcs, 0);
     tock();
-    TLAUNCH("bucket_scatter", mps::k_bkt_scatter, dim3(nblk(nf)), B, 0, stream, key, slot, fcs, nf,
-            (const int*)nullptr, 1, tmp);
-    TLAUNCH("bucket_rank", mps::k_bkt_rank, dim3(nblk(nf)), B, 0, stream, tmp, key, fid[0], fcs, nf,
+    TLAUNCH("bucket_scatter", mps::k_bkt_scatter, dim3(nblk(nf)), B, 0, stream, key, slot, fid[0], fcs,
+            nf, (const int*)nullptr, 1, tmp, tmp_id);
+    TLAUNCH("bucket_rank", mps::k_bkt_rank, dim3(nblk(nf)), B, 0, stream, tmp, tmp_id, key, fcs, nf,
             (const int*)nullptr, tot, perm);
     TLAUNCH("bucket_gather", (mps::k_gather_fluid<real, DIM>), dim3(nblk(nf)), B, 0, stream, perm, nf,
             fa(0), fa(1));
@@ -536,10 +537,10 @@ struct Impl : ImplBase {
     tick("mirror_scan");
     scan(cnt, tot + 1, mcs, 0);
     tock();
-    TLAUNCH("mirror_scatter", mps::k_bkt_scatter, dim3(g), B, 0, stream, (const int*)mraw_key, slot, mcs,
-            0, (const int*)mcount, mult, tmp);
+    TLAUNCH("mirror_scatter", mps::k_bkt_scatter, dim3(g), B, 0, stream, (const int*)mraw_key, slot,
+            mraw_id, mcs, 0, (const int*)mcount, mult, tmp, tmp_id);
     // live mirrors: mcs[tot]
-    TLAUNCH("mirror_rank", mps::k_bkt_rank, dim3(g), B, 0, stream, tmp, (const int*)mraw_key, mraw_id, mcs,
+    TLAUNCH("mirror_rank", mps::k_bkt_rank, dim3(g), B, 0, stream, tmp, tmp_id, (const int*)mraw_key, mcs,
             0, (const int*)(mcs + tot), tot, perm);
     TLAUNCH("mirror_gather", (mps::k_gather_mirror<real, DIM>), dim3(g), B, 0, stream, perm,
             (const int*)(mcs + tot), mraw_pos, mraw_prs, mraw_id, mpos, mprs, mid);

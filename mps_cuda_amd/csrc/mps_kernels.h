@@ -209,6 +209,27 @@ __global__ void k_cell_near(const int* __restrict__ cs, const Cst<real>* __restr
 // ---------------------------------------------------------------------------
 // Bucket build: key + count, scatter, rank inside the cell by id, gather.
 
+// slot = atomicAdd(&cnt[k], 1) with one atomic per distinct key of the
+// wavefront (sorted input gives a few keys per wave, and same-address
+// atomics serialize).  Slots within a cell only order the scatter; the
+// rank kernel fixes the final order.  Call with the lanes that count.
+__device__ inline int wave_count(int* cnt, int k) {
+  const int lane = __lane_id();
+  unsigned long long todo = __ballot(1);
+  int slot = 0;
+  while (todo) {
+    const int leader = __ffsll((long long)todo) - 1;
+    const int lk = __shfl(k, leader);
+    const unsigned long long same = __ballot(k == lk);
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&cnt[lk], __popcll(same));
+    base = __shfl(base, leader);
+    if (k == lk) slot = base + __popcll(same & ((1ull << lane) - 1ull));
+    todo &= ~same;
+  }
+  return slot;
+}
+
 template <class real, int DIM>
 __global__ void k_bkt_key(const Vec<real, DIM>* __restrict__ pos, const int8_t* __restrict__ typ,
                           const int* __restrict__ sid, const Cst<real>* __restrict__ tab, int n,
@@ -222,7 +243,7 @@ __global__ void k_bkt_key(const Vec<real, DIM>* __restrict__ pos, const int8_t* 
       k = cell_key(q.ix, q.iy, q.iz, c);
     }
     key[i] = k;
-    slot[i] = atomicAdd(&cnt[k], 1);
+    slot[i] = wave_count(cnt, k);
   }
 }
 
@@ -232,26 +253,34 @@ __global__ void k_key_count(const int* __restrict__ key, const int* __restrict__
   const int n = *n_dyn * mult;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const int k = key[i];
-    if (k >= 0) slot[i] = atomicAdd(&cnt[k], 1);
+    if (k >= 0) slot[i] = wave_count(cnt, k);
   }
 }
 
-// Element count: n_static, or mult * (*n_dyn) when n_dyn is given.
+// Element count: n_static, or mult * (*n_dyn) when n_dyn is given.  The
+// element's id goes beside it (tmp_id) for the rank kernel.
 __global__ void k_bkt_scatter(const int* __restrict__ key, const int* __restrict__ slot,
-                              const int* __restrict__ cs, int n_static, const int* __restrict__ n_dyn,
-                              int mult, int* __restrict__ tmp) {
+                              const int* __restrict__ id, const int* __restrict__ cs, int n_static,
+                              const int* __restrict__ n_dyn, int mult, int* __restrict__ tmp,
+                              int* __restrict__ tmp_id) {
   const int n = n_dyn ? *n_dyn * mult : n_static;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const int k = key[i];
-    if (k >= 0) tmp[cs[k] + slot[i]] = i;
+    if (k >= 0) {
+      const int o = cs[k] + slot[i];
+      tmp[o] = i;
+      tmp_id[o] = id[i];
+    }
   }
 }
 
 // Final position of an element = start of its cell + number of elements of
-// the same cell with a smaller id.  The deleted tail (key == tot) keeps the
-// scatter order: its order is never observed.
-__global__ void k_bkt_rank(const int* __restrict__ tmp, const int* __restrict__ key,
-                           const int* __restrict__ id, const int* __restrict__ cs, int n_static,
+// the same cell with a smaller id (the cell's ids are read eight at a time
+// from tmp_id; past the cell's end the first id is read and not counted).
+// The deleted tail (key == tot) keeps the scatter order: its order is never
+// observed.
+__global__ void k_bkt_rank(const int* __restrict__ tmp, const int* __restrict__ tmp_id,
+                           const int* __restrict__ key, const int* __restrict__ cs, int n_static,
                            const int* __restrict__ n_dyn, int tot, int* __restrict__ perm) {
   const int n = n_dyn ? *n_dyn : n_static;
   for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
@@ -262,9 +291,15 @@ __global__ void k_bkt_rank(const int* __restrict__ tmp, const int* __restrict__ 
       continue;
     }
     const int b = cs[k], e = cs[k + 1];
-    const int me = id[i];
+    const int me = tmp_id[s];
     int rank = 0;
-    for (int t = b; t < e; t++) rank += id[tmp[t]] < me;
+    for (int t = b; t < e; t += 8) {
+      int v[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) v[u] = tmp_id[t + u < e ? t + u : b];
+#pragma unroll
+      for (int u = 0; u < 8; u++) rank += (t + u < e) & (v[u] < me);
+    }
     perm[b + rank] = i;
   }
 }
