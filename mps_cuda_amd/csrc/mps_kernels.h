@@ -92,6 +92,53 @@ __device__ inline double near2(double x, double y, double z) {
 }
 
 // ---------------------------------------------------------------------------
+// Wave-staged row scans.  In a sorted set the lanes of a wavefront mostly
+// share their stencil rows; a lane that loads candidate j itself makes the
+// wavefront issue one 64-lane load per candidate, all for the same few
+// addresses, and those loads, not the arithmetic, bound the scan.
+// scan_row_staged loads the union [B, E) of the lanes' ranges of one row
+// once, 64 candidates per load, into the wavefront's LDS stage, and each
+// lane runs scan_chunks on its own range from there.  A row whose union
+// exceeds the stage (a wavefront spanning two bucket rows) is scanned from
+// global memory.  Every lane of the wavefront must call it (a lane without
+// a particle passes act = false); test(v) sees the candidate's value; the
+// visiting order is that of scan_chunks on [b, e).  LDS accesses of one
+// wavefront complete in issue order, so the stage needs no barrier beyond
+// keeping the compiler from moving accesses across the two phases.
+__device__ inline int wave_min(int v) {
+  for (int off = 32; off >= 1; off >>= 1) v = min(v, __shfl_xor(v, off));
+  return v;
+}
+__device__ inline int wave_max(int v) {
+  for (int off = 32; off >= 1; off >>= 1) v = max(v, __shfl_xor(v, off));
+  return v;
+}
+
+constexpr int kStageBytes = 7168;  // per wavefront
+template <class V>
+constexpr int stage_cap() {
+  return kStageBytes / int(sizeof(V)) - 8;  // 8 spare entries for the scan's overrun
+}
+
+template <class V, class Test, class Emit>
+__device__ inline void scan_row_staged(V* stage, int b, int e, bool act, const V* __restrict__ arr,
+                                       Test&& test, Emit&& emit) {
+  const int B = wave_min(act ? b : 0x7fffffff), E = wave_max(act ? e : -1);
+  if (E <= B) return;
+  if (E - B <= stage_cap<V>()) {
+    for (int t = __lane_id(); t < E - B; t += 64) stage[t] = arr[B + t];
+    __builtin_amdgcn_wave_barrier();
+    if (act) {
+      const V* s = stage - B;
+      scan_chunks(b, e, [&](int j) { return test(s[j]); }, emit);
+    }
+    __builtin_amdgcn_wave_barrier();
+  } else if (act) {
+    scan_chunks(b, e, [&](int j) { return test(arr[j]); }, emit);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Exclusive scan of int counts (cell occupancy -> cell_start).
 // Tile = kBlock threads x kScanIPT items.
 constexpr int kScanIPT = 8;
@@ -846,11 +893,15 @@ __global__ __launch_bounds__(kBlock) void k_vsc(
     const int* __restrict__ mcs, const uint8_t* __restrict__ mnear, Chunk* chunks, int n_alloc,
     Vec<real, DIM>* __restrict__ acc_out, Vec<real, DIM>* __restrict__ pos_out,
     Vec<real, DIM>* __restrict__ vel_out, int8_t* __restrict__ typ, real* __restrict__ prs) {
+  using V = Vec<real, DIM>;
+  __shared__ V s_stage[kBlock / 64][stage_cap<V>() + 8];
   int s, first, end;
   Cst<real> c;
-  if (!scene_block(sc, fcs, s, c, first, end)) return;
-  const int i = first + threadIdx.x;
-  if (i >= end) return;
+  if (!scene_block(sc, fcs, s, c, first, end)) return;  // whole workgroup
+  // every lane stays for the wave-wide staging; act marks a particle
+  const bool act = first + int(threadIdx.x) < end;
+  const int i = act ? first + int(threadIdx.x) : first;
+  V* stage = s_stage[threadIdx.x / 64];
   const Vec<real, DIM> p = pos[i], v = vel[i];
   const real px = p.x, py = p.y, pz = vz(p);
   const real vx = v.x, vy = v.y, vvz = vz(v);
@@ -898,17 +949,15 @@ __global__ __launch_bounds__(kBlock) void k_vsc(
   for (int jz = zlo; jz <= zhi; jz++)
     for (int jy = q.iy - 1; jy <= q.iy + 1; jy++) {
       const int b = fcs[cell_key(q.ix - 1, jy, jz, c)], e = fcs[cell_key(q.ix + 1, jy, jz, c) + 1];
-      scan_chunks(
-          b, e,
-          [&](int j) {
-            const Vec<real, DIM> pj = pos[j];
-            return near2(pj.x - px, pj.y - py, vz(pj) - pz) <= r2_hi;
-          },
+      scan_row_staged(
+          stage, b, e, act, pos,
+          [&](const V& pj) { return near2(pj.x - px, pj.y - py, vz(pj) - pz) <= r2_hi; },
           [&](int base, unsigned long long m) {
             if (unsigned(i - base) < 64u) m &= ~(1ull << (i - base));  // j != i
             if (m) emit(base, m);
           });
     }
+  if (!act) return;
   if (mnear[cell_key(q.ix, q.iy, q.iz, c)])
     for (int jz = zlo; jz <= zhi; jz++)
       for (int jy = q.iy - 1; jy <= q.iy + 1; jy++) {

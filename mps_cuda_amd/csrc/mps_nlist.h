@@ -22,11 +22,15 @@ __global__ __launch_bounds__(kBlock) void k_nlist_build(const Vec<real, DIM>* __
                                                        const int* __restrict__ fcs, int n_alloc,
                                                        Scenes<real> sc, int* __restrict__ list,
                                                        int* __restrict__ count) {
+  using V = Vec<real, DIM>;
+  __shared__ V s_stage[kBlock / 64][stage_cap<V>() + 8];
   int s, first, end;
   Cst<real> c;
-  if (!scene_block(sc, fcs, s, c, first, end)) return;
-  const int i = first + threadIdx.x;
-  if (i >= end) return;
+  if (!scene_block(sc, fcs, s, c, first, end)) return;  // whole workgroup
+  // every lane stays for the wave-wide staging; act marks a particle
+  const bool act = first + int(threadIdx.x) < end;
+  const int i = act ? first + int(threadIdx.x) : first;
+  V* stage = s_stage[threadIdx.x / 64];
   const Vec<real, DIM> p = pos[i];
   const real px = p.x, py = p.y, pz = vz(p), lim = c.rp2_hi;
   const Cell<real, DIM> q = cell_of<real, DIM>(p, c);
@@ -35,12 +39,9 @@ __global__ __launch_bounds__(kBlock) void k_nlist_build(const Vec<real, DIM>* __
   for (int jz = zlo; jz <= zhi; jz++)
     for (int jy = q.iy - 1; jy <= q.iy + 1; jy++) {
       const int b = fcs[cell_key(q.ix - 1, jy, jz, c)], e = fcs[cell_key(q.ix + 1, jy, jz, c) + 1];
-      scan_chunks(
-          b, e,
-          [&](int j) {
-            const Vec<real, DIM> pj = pos[j];
-            return near2(pj.x - px, pj.y - py, vz(pj) - pz) <= lim;
-          },
+      scan_row_staged(
+          stage, b, e, act, pos,
+          [&](const V& pj) { return near2(pj.x - px, pj.y - py, vz(pj) - pz) <= lim; },
           [&](int base, unsigned long long m) {
             if (unsigned(i - base) < 64u) m &= ~(1ull << (i - base));  // j != i
             for_bits(base, m, [&](int j) {
@@ -49,7 +50,7 @@ __global__ __launch_bounds__(kBlock) void k_nlist_build(const Vec<real, DIM>* __
             });
           });
     }
-  count[i] = n <= kNList ? n : -1;
+  if (act) count[i] = n <= kNList ? n : -1;
 }
 
 // Calls f(j) for the fluid candidates of particle i in visiting order: from
