@@ -957,19 +957,17 @@ __global__ __launch_bounds__(kBlock) void k_vsc(
             if (m) emit(base, m);
           });
     }
-  if (!act) return;
-  if (mnear[cell_key(q.ix, q.iy, q.iz, c)])
+  const bool mact = act && mnear[cell_key(q.ix, q.iy, q.iz, c)];
+  if (__ballot(mact))
     for (int jz = zlo; jz <= zhi; jz++)
       for (int jy = q.iy - 1; jy <= q.iy + 1; jy++) {
         const int b = mcs[cell_key(q.ix - 1, jy, jz, c)], e = mcs[cell_key(q.ix + 1, jy, jz, c) + 1];
-        scan_chunks(
-            b, e,
-            [&](int j) {
-              const Vec<real, DIM> pj = mpos[j];
-              return near2(pj.x - px, pj.y - py, vz(pj) - pz) <= r2_hi;
-            },
+        scan_row_staged(
+            stage, b, e, mact, mpos,
+            [&](const V& pj) { return near2(pj.x - px, pj.y - py, vz(pj) - pz) <= r2_hi; },
             [&](int base, unsigned long long m) { emit(base | kMirrorChunk, m); });
       }
+  if (!act) return;
   walk();
   const real accx = c.n0knm * ax + c.g[0];
   const real accy = c.n0knm * ay + c.g[1];

@@ -122,23 +122,24 @@ __device__ inline void for_near(int i, real px, real py, real pz, size_t n_alloc
     }
 }
 
-// Mirror candidates of the stencil within rp (with margin), in order.
+// Mirror candidates of the stencil within rp (with margin), in order, for
+// the lanes with act set; the stencil rows are staged per wavefront, so
+// every lane of the wavefront calls it.  Skipped when no lane needs it.
 template <class real, int DIM, class F>
-__device__ inline void for_mirrors_near(real px, real py, real pz, const Vec<real, DIM>* __restrict__ mpos,
+__device__ inline void for_mirrors_near(Vec<real, DIM>* stage, bool act, real px, real py, real pz,
+                                        const Vec<real, DIM>* __restrict__ mpos,
                                         const int* __restrict__ mcs, const Cell<real, DIM>& q,
                                         const Cst<real>& c, F&& f) {
+  if (!__ballot(act)) return;
   const int zlo = DIM == 3 ? q.iz - 1 : 0, zhi = DIM == 3 ? q.iz + 1 : 0;
   const real lim = c.rp2_hi;
   for (int jz = zlo; jz <= zhi; jz++)
     for (int jy = q.iy - 1; jy <= q.iy + 1; jy++) {
       const int b = mcs[cell_key(q.ix - 1, jy, jz, c)], e = mcs[cell_key(q.ix + 1, jy, jz, c) + 1];
-      for_hits(
-          b, e,
-          [&](int j) {
-            const Vec<real, DIM> pj = mpos[j];
-            return near2(pj.x - px, pj.y - py, vz(pj) - pz) <= lim;
-          },
-          [&](int j) { f(j, mpos[j]); });
+      scan_row_staged(
+          stage, b, e, act, mpos,
+          [&](const Vec<real, DIM>& pj) { return near2(pj.x - px, pj.y - py, vz(pj) - pz) <= lim; },
+          [&](int base, unsigned long long m) { for_bits(base, m, [&](int j) { f(j, mpos[j]); }); });
     }
 }
 
@@ -162,11 +163,15 @@ __global__ __launch_bounds__(kBlock) void k_colprs_list(
     const Vec<real, DIM>* __restrict__ mpos, const int* __restrict__ mcs,
     const uint8_t* __restrict__ wnear, const uint8_t* __restrict__ mnear, Scenes<real> sc,
     Vec<real, DIM>* __restrict__ vel_out, real* __restrict__ prs, real* __restrict__ nd) {
+  using V = Vec<real, DIM>;
+  __shared__ V s_stage[kBlock / 64][stage_cap<V>() + 8];
   int s, first, end;
   Cst<real> c;
-  if (!scene_block(sc, fcs, s, c, first, end)) return;
-  const int i = first + threadIdx.x;
-  if (i >= end) return;
+  if (!scene_block(sc, fcs, s, c, first, end)) return;  // whole workgroup
+  // every lane stays for the wave-wide staging; act marks a particle
+  const bool act = first + int(threadIdx.x) < end;
+  const int i = act ? first + int(threadIdx.x) : first;
+  V* stage = s_stage[threadIdx.x / 64];
   const Vec<real, DIM> p = pos[i], v = vel[i];
   const real px = p.x, py = p.y, pz = vz(p);
   const real vx = v.x, vy = v.y, vvz = vz(v);
@@ -190,9 +195,10 @@ __global__ __launch_bounds__(kBlock) void k_colprs_list(
   };
   const Cell<real, DIM> q = cell_of<real, DIM>(p, c);
   const int hk = cell_key(q.ix, q.iy, q.iz, c);
-  const bool walls = wnear[hk];
+  const bool walls = act && wnear[hk];
   const int n = count[i];
   int fluid_contacts = 0;
+  if (act)
   for_near<real, DIM>(i, px, py, pz, (size_t)n_alloc, list, n, pos, fcs, q, c,
                       [&](int j, const Vec<real, DIM>& pj, real) {
                         const real qx = pj.x - px, qy = pj.y - py, qz = vz(pj) - pz;
@@ -206,22 +212,23 @@ __global__ __launch_bounds__(kBlock) void k_colprs_list(
                           fluid_contacts++;
                         }
                       });
-  if (walls) {
-    const int zlo = DIM == 3 ? q.iz - 1 : 0, zhi = DIM == 3 ? q.iz + 1 : 0;
-    if (fluid_contacts == 0) {
-      const real wl = c.rlim_wall2;
-      for (int jz = zlo; jz <= zhi; jz++)
-        for (int jy = q.iy - 1; jy <= q.iy + 1; jy++) {
-          const int b = wcs[cell_key(q.ix - 1, jy, jz, c)], e = wcs[cell_key(q.ix + 1, jy, jz, c) + 1];
-          for_hits(
-              b, e,
-              [&](int j) {
-                const Vec<real, DIM> pj = wpos[j];
-                return norm2<real, DIM>(pj.x - px, pj.y - py, vz(pj) - pz) < wl;
-              },
-              [&](int j) { contact(wpos[j], [&] { return wvel[j]; }, wl); });
-        }
-    } else {
+  const int zlo = DIM == 3 ? q.iz - 1 : 0, zhi = DIM == 3 ? q.iz + 1 : 0;
+  const bool wscan = walls && fluid_contacts == 0;
+  if (__ballot(wscan)) {
+    const real wl = c.rlim_wall2;
+    for (int jz = zlo; jz <= zhi; jz++)
+      for (int jy = q.iy - 1; jy <= q.iy + 1; jy++) {
+        const int b = wcs[cell_key(q.ix - 1, jy, jz, c)], e = wcs[cell_key(q.ix + 1, jy, jz, c) + 1];
+        scan_row_staged(
+            stage, b, e, wscan, wpos,
+            [&](const V& pj) { return norm2<real, DIM>(pj.x - px, pj.y - py, vz(pj) - pz) < wl; },
+            [&](int base, unsigned long long m) {
+              for_bits(base, m, [&](int j) { contact(wpos[j], [&] { return wvel[j]; }, wl); });
+            });
+      }
+  }
+  if (walls && fluid_contacts > 0) {
+    {
       int k = 0;
       for (int jz = zlo; jz <= zhi; jz++)
         for (int jy = q.iy - 1; jy <= q.iy + 1; jy++)
@@ -243,8 +250,8 @@ __global__ __launch_bounds__(kBlock) void k_colprs_list(
           }
     }
   }
-  if (mnear[hk])
-    for_mirrors_near<real, DIM>(px, py, pz, mpos, mcs, q, c, [&](int, const Vec<real, DIM>& pj) {
+  for_mirrors_near<real, DIM>(stage, act && mnear[hk], px, py, pz, mpos, mcs, q, c,
+                              [&](int, const Vec<real, DIM>& pj) {
       const real qx = pj.x - px, qy = pj.y - py, qz = vz(pj) - pz;
       const real d2 = norm2<real, DIM>(qx, qy, qz);
       if (d2 <= lim) {
@@ -253,6 +260,7 @@ __global__ __launch_bounds__(kBlock) void k_colprs_list(
       }
       if (d2 < rlim2) apply(qx, qy, qz, d2, mkvec<real, DIM>(0, 0, 0));
     });
+  if (!act) return;
   vel_out[i] = mkvec<real, DIM>(v2x, v2y, v2z);
   real pressure = real(ni > c.n0_grad) * (ni - c.n0_grad) * c.prs_coef * c.mass;
   if (pressure <= c.pmax && pressure >= real(0)) {
@@ -395,11 +403,15 @@ __global__ __launch_bounds__(kBlock) void k_grad_list(
     Vec<real, DIM>* __restrict__ acc_out, Vec<real, DIM>* __restrict__ pos_out,
     Vec<real, DIM>* __restrict__ vel_out, real* __restrict__ prs_out, real* __restrict__ pav,
     int8_t* __restrict__ typ) {
+  using V = Vec<real, DIM>;
+  __shared__ V s_stage[kBlock / 64][stage_cap<V>() + 8];
   int s, first, end;
   Cst<real> c;
-  if (!scene_block(sc, fcs, s, c, first, end)) return;
-  const int i = first + threadIdx.x;
-  if (i >= end) return;
+  if (!scene_block(sc, fcs, s, c, first, end)) return;  // whole workgroup
+  // every lane stays for the wave-wide staging; act marks a particle
+  const bool act = first + int(threadIdx.x) < end;
+  const int i = act ? first + int(threadIdx.x) : first;
+  V* stage = s_stage[threadIdx.x / 64];
   const Vec<real, DIM> p = pos[i];
   const real px = p.x, py = p.y, pz = vz(p), rp = c.rp, rp2 = c.rp2;
   const real prs_i = prs[i];
@@ -408,6 +420,7 @@ __global__ __launch_bounds__(kBlock) void k_grad_list(
   const Cell<real, DIM> q = cell_of<real, DIM>(p, c);
   const int n = count[i];
   // P_min over fluid neighbours with d2 < rp2 (kernel.cu:1686-1700)
+  if (act)
   for_near<real, DIM>(
       i, px, py, pz, (size_t)n_alloc, list, n, pos, fcs, q, c,
       [&](int, const Vec<real, DIM>& pj, real pr) {
@@ -425,12 +438,13 @@ __global__ __launch_bounds__(kBlock) void k_grad_list(
     ay += d * w * qy;
     if constexpr (DIM == 3) az += d * w * qz;
   };
+  if (act)
   for_near<real, DIM>(
       i, px, py, pz, (size_t)n_alloc, list, n, pos, fcs, q, c,
       [&](int, const Vec<real, DIM>& pj, real pr) { add(pj, pr); }, prs);
-  if (mnear[cell_key(q.ix, q.iy, q.iz, c)])
-    for_mirrors_near<real, DIM>(px, py, pz, mpos, mcs, q, c,
-                                [&](int j, const Vec<real, DIM>& pj) { add(pj, mprs[j]); });
+  for_mirrors_near<real, DIM>(stage, act && mnear[cell_key(q.ix, q.iy, q.iz, c)], px, py, pz, mpos,
+                              mcs, q, c, [&](int j, const Vec<real, DIM>& pj) { add(pj, mprs[j]); });
+  if (!act) return;
   const real accx = c.invro * ax * c.grad_a;
   const real accy = c.invro * ay * c.grad_a;
   const real accz = DIM == 3 ? c.invro * az * c.grad_a : real(0);
