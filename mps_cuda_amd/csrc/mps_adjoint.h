@@ -133,17 +133,36 @@ __global__ void k_adj_out(const int* __restrict__ fid, int nf, long long n_total
   }
 }
 
-// *acc += sum of a[0 .. cs[tot]) in a fixed order (one workgroup).
+// *acc += sum of a[0 .. cs[tot]) in a fixed order: k_sum_parts with a
+// fixed grid of kSumParts workgroups writes one partial per workgroup (its
+// contiguous slice, strided per thread, then a fixed tree), k_sum_final
+// adds the partials in a fixed tree.  The result does not depend on timing.
+constexpr int kSumParts = 256;
+
 template <class real>
-__global__ __launch_bounds__(1024) void k_sum_into(const real* __restrict__ a, const int* __restrict__ cs,
-                                                   int tot, double* __restrict__ acc) {
-  __shared__ double sh[1024];
+__global__ __launch_bounds__(kBlock) void k_sum_parts(const real* __restrict__ a, const int* __restrict__ cs,
+                                                      int tot, double* __restrict__ part) {
+  __shared__ double sh[kBlock];
   const int n = cs[tot];
+  const int L = (n + kSumParts - 1) / kSumParts;
+  const int b0 = blockIdx.x * L, b1 = min(n, b0 + L);
   double s = 0;
-  for (int i = threadIdx.x; i < n; i += 1024) s += double(a[i]);
+  for (int i = b0 + threadIdx.x; i < b1; i += kBlock) s += double(a[i]);
   sh[threadIdx.x] = s;
   __syncthreads();
-  for (int w = 512; w > 0; w >>= 1) {
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ __launch_bounds__(kSumParts) void k_sum_final(const double* __restrict__ part,
+                                                         double* __restrict__ acc) {
+  __shared__ double sh[kSumParts];
+  sh[threadIdx.x] = part[threadIdx.x];
+  __syncthreads();
+  for (int w = kSumParts / 2; w > 0; w >>= 1) {
     if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
     __syncthreads();
   }
@@ -204,24 +223,29 @@ __global__ __launch_bounds__(kBlock) void k_adj_grad_pre(
   real pmin = prs[i];
   int am = i;
   const Cell<real, DIM> q = cell_of<real, DIM>(pos[i], c);
-  for_fluid_near<real, DIM>(i, nf, list, count, fcs, q, c, [&](int j) {
-    real y[3];
-    ld3(pos[j], y);
-    if (norm2<real, DIM>(y[0] - x[0], y[1] - x[1], y[2] - x[2]) < rp2 && prs[j] < pmin) {
-      pmin = prs[j];
-      am = j;
-    }
-  });
+  const int n = count[i];
+  for_near<real, DIM>(
+      i, x[0], x[1], x[2], (size_t)nf, list, n, pos, fcs, q, c,
+      [&](int j, const Vec<real, DIM>& pj, real pr) {
+        real y[3];
+        ld3(pj, y);
+        if (norm2<real, DIM>(y[0] - x[0], y[1] - x[1], y[2] - x[2]) < rp2 && pr < pmin) {
+          pmin = pr;
+          am = j;
+        }
+      },
+      prs);
   real s = 0;
-  for_fluid_near<real, DIM>(i, nf, list, count, fcs, q, c, [&](int j) {
-    real y[3], p[3];
-    ld3(pos[j], y);
-    for (int k = 0; k < 3; k++) p[k] = y[k] - x[k];
-    const real d2 = norm2<real, DIM>(p[0], p[1], p[2]);
-    if (!(d2 < rp2)) return;
-    const real d = sqrt(d2);
-    s += ((rp / d - d / rp) / d2) * dotv<real, DIM>(p, a);
-  });
+  for_near<real, DIM>(i, x[0], x[1], x[2], (size_t)nf, list, n, pos, fcs, q, c,
+                      [&](int, const Vec<real, DIM>& pj, real) {
+                        real y[3], p[3];
+                        ld3(pj, y);
+                        for (int k = 0; k < 3; k++) p[k] = y[k] - x[k];
+                        const real d2 = norm2<real, DIM>(p[0], p[1], p[2]);
+                        if (!(d2 < rp2)) return;
+                        const real d = sqrt(d2);
+                        s += ((rp / d - d / rp) / d2) * dotv<real, DIM>(p, a);
+                      });
   real xacc[3] = {0, 0, 0};
   if (mnear[cell_key(q.ix, q.iy, q.iz, c)])
     for_stencil<real, DIM>(q, c, mcs, [&](int m) {
@@ -268,9 +292,10 @@ __global__ __launch_bounds__(kBlock) void k_adj_grad(
   const real Pk = prs[k], Pmk = pmin[k];
   real xacc[3] = {0, 0, 0}, pacc = 0;
   const Cell<real, DIM> q = cell_of<real, DIM>(pos[k], c);
-  for_fluid_near<real, DIM>(k, nf, list, count, fcs, q, c, [&](int j) {
+  for_near<real, DIM>(k, x[0], x[1], x[2], (size_t)nf, list, count[k], pos, fcs, q, c,
+                      [&](int j, const Vec<real, DIM>& pj, real prj) {
     real y[3], p[3], aj[3];
-    ld3(pos[j], y);
+    ld3(pj, y);
     for (int t = 0; t < 3; t++) p[t] = y[t] - x[t];
     const real d2 = norm2<real, DIM>(p[0], p[1], p[2]);
     if (!(d2 < rp2)) return;
@@ -279,12 +304,12 @@ __global__ __launch_bounds__(kBlock) void k_adj_grad(
     const real f = (rp / d - d / rp) / d2;
     const real fp = real(-3) * rp / (d2 * d2) + real(1) / (rp * d2);
     const real pak = dotv<real, DIM>(p, a), paj = dotv<real, DIM>(p, aj);
-    const real t1 = K * (prs[j] - Pmk), t2 = K * (Pk - pmin[j]);
+    const real t1 = K * (prj - Pmk), t2 = K * (Pk - pmin[j]);
     for (int t = 0; t < 3; t++)
       xacc[t] += -t1 * (f * a[t] + fp / d * pak * p[t]) + t2 * (f * aj[t] + fp / d * paj * p[t]);
     pacc -= K * f * paj;
     if (argmin[j] == k) pacc -= sg[j];
-  });
+  }, prs);
   if (argmin[k] == k) pacc -= sg[k];
   real xo[3];
   ld3(xb[k], xo);
@@ -357,17 +382,20 @@ __global__ __launch_bounds__(kBlock) void k_adj_prs(
   const real rp = c.rp, nk = nbar[k];
   real xacc[3] = {0, 0, 0};
   const Cell<real, DIM> q = cell_of<real, DIM>(pos[k], c);
-  for_fluid_near<real, DIM>(k, nf, list, count, fcs, q, c, [&](int j) {
-    real y[3], p[3];
-    ld3(pos[j], y);
-    for (int t = 0; t < 3; t++) p[t] = y[t] - x[t];
-    const real d2 = norm2<real, DIM>(p[0], p[1], p[2]);
-    if (!(d2 <= c.rp2_hi)) return;
-    const real d = sqrt(d2);
-    if (!(d < rp)) return;
-    const real co = (nk + nbar[j]) * (-rp / d2 - real(1) / rp) / d;
-    for (int t = 0; t < 3; t++) xacc[t] -= co * p[t];
-  });
+  for_near<real, DIM>(
+      k, x[0], x[1], x[2], (size_t)nf, list, count[k], pos, fcs, q, c,
+      [&](int, const Vec<real, DIM>& pj, real nbj) {
+        real y[3], p[3];
+        ld3(pj, y);
+        for (int t = 0; t < 3; t++) p[t] = y[t] - x[t];
+        const real d2 = norm2<real, DIM>(p[0], p[1], p[2]);
+        if (!(d2 <= c.rp2_hi)) return;
+        const real d = sqrt(d2);
+        if (!(d < rp)) return;
+        const real co = (nk + nbj) * (-rp / d2 - real(1) / rp) / d;
+        for (int t = 0; t < 3; t++) xacc[t] -= co * p[t];
+      },
+      nbar);
   if (mnear[cell_key(q.ix, q.iy, q.iz, c)])
     for_stencil<real, DIM>(q, c, mcs, [&](int m) {
       real y[3], p[3];
@@ -428,9 +456,10 @@ __global__ __launch_bounds__(kBlock) void k_adj_col(
     return true;
   };
   const Cell<real, DIM> q = cell_of<real, DIM>(pos[k], c);
-  for_fluid_near<real, DIM>(k, nf, list, count, fcs, q, c, [&](int j) {
+  for_near<real, DIM>(k, x[0], x[1], x[2], (size_t)nf, list, count[k], pos, fcs, q, c,
+                      [&](int j, const Vec<real, DIM>& pj, real) {
     real y[3], vj[3], gj[3], p[3], u[3], gd[3], F[3];
-    ld3(pos[j], y);
+    ld3(pj, y);
     for (int t = 0; t < 3; t++) p[t] = y[t] - x[t];
     if (!(norm2<real, DIM>(p[0], p[1], p[2]) < c.rlim2)) return;
     ld3(vel[j], vj);
@@ -442,17 +471,29 @@ __global__ __launch_bounds__(kBlock) void k_adj_col(
     contact(p, u, gd, c.rlim2, F);
   });
   const int hk = cell_key(q.ix, q.iy, q.iz, c);
-  if (wnear[hk])
-    for_stencil<real, DIM>(q, c, wcs, [&](int j) {
-      real y[3], vj[3], p[3], u[3], F[3];
-      ld3(wpos[j], y);
-      ld3(wvel[j], vj);
-      for (int t = 0; t < 3; t++) {
-        p[t] = y[t] - x[t];
-        u[t] = v[t] - vj[t];
+  if (wnear[hk]) {  // walls: only those within the contact radius do anything
+    const int zlo = DIM == 3 ? q.iz - 1 : 0, zhi = DIM == 3 ? q.iz + 1 : 0;
+    for (int jz = zlo; jz <= zhi; jz++)
+      for (int jy = q.iy - 1; jy <= q.iy + 1; jy++) {
+        const int b = wcs[cell_key(q.ix - 1, jy, jz, c)], e = wcs[cell_key(q.ix + 1, jy, jz, c) + 1];
+        for_hits(
+            b, e,
+            [&](int j) {
+              const Vec<real, DIM> w = wpos[j];
+              return norm2<real, DIM>(w.x - x[0], w.y - x[1], vz(w) - x[2]) < c.rlim_wall2;
+            },
+            [&](int j) {
+              real y[3], vj[3], p[3], u[3], F[3];
+              ld3(wpos[j], y);
+              ld3(wvel[j], vj);
+              for (int t = 0; t < 3; t++) {
+                p[t] = y[t] - x[t];
+                u[t] = v[t] - vj[t];
+              }
+              contact(p, u, g, c.rlim_wall2, F);
+            });
       }
-      contact(p, u, g, c.rlim_wall2, F);
-    });
+  }
   if (mnear[hk])
     for_stencil<real, DIM>(q, c, mcs, [&](int m) {
       real y[3], p[3], F[3];
@@ -511,15 +552,23 @@ __global__ void k_adj_up1(const int* __restrict__ cs, int tot, int nf, const Vec
 // Gathers vbar_k += kv sum h (abar_j - abar_k) - kv sum_m h abar_k,
 // xbar_k -= kv h'(r)/r ((abar_k - abar_j) . (v_j - v_k)) p, the mirror
 // position terms, and nupart_k = abar_k . S_k / n0 (S_k the bracket).
+// The fluid neighbours are found as in the forward VscTrm: wave-staged
+// stencil rows recorded as 64-candidate chunks, then walked per lane.
 template <class real, int DIM>
 __global__ __launch_bounds__(kBlock) void k_adj_vsc(
     const Vec<real, DIM>* __restrict__ pos, const Vec<real, DIM>* __restrict__ vel,
     const int* __restrict__ fcs, const Vec<real, DIM>* __restrict__ mpos, const int* __restrict__ mcs,
     const uint8_t* __restrict__ mnear, const Vec<real, DIM>* __restrict__ ab, Cst<real> c, real inv_n0,
-    Vec<real, DIM>* __restrict__ xb, Vec<real, DIM>* __restrict__ vb, Vec<real, DIM>* __restrict__ yb,
-    real* __restrict__ nupart) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= fcs[c.tot]) return;
+    Chunk* chunks, int n_alloc, Vec<real, DIM>* __restrict__ xb, Vec<real, DIM>* __restrict__ vb,
+    Vec<real, DIM>* __restrict__ yb, real* __restrict__ nupart) {
+  using V = Vec<real, DIM>;
+  __shared__ V s_stage[kBlock / 64][stage_cap<V>() + 8];
+  const int n_live = fcs[c.tot];
+  if (int(blockIdx.x * blockDim.x) >= n_live) return;  // whole workgroup
+  // every lane stays for the wave-wide staging; act marks a particle
+  const bool act = int(blockIdx.x * blockDim.x + threadIdx.x) < n_live;
+  const int k = act ? int(blockIdx.x * blockDim.x + threadIdx.x) : 0;
+  V* stage = s_stage[threadIdx.x / 64];
   real x[3], v[3], a[3];
   ld3(pos[k], x);
   ld3(vel[k], v);
@@ -527,37 +576,50 @@ __global__ __launch_bounds__(kBlock) void k_adj_vsc(
   const real kn = c.n0knm, l0 = c.l0, r = c.r;
   real vacc[3] = {0, 0, 0}, xacc[3] = {0, 0, 0}, S[3] = {0, 0, 0};
   const Cell<real, DIM> q = cell_of<real, DIM>(pos[k], c);
+  auto body = [&](int j) {
+    real y[3], p[3], vj[3], aj[3], dv[3], da[3];
+    ld3(pos[j], y);
+    for (int t = 0; t < 3; t++) p[t] = y[t] - x[t];
+    const real d2 = norm2<real, DIM>(p[0], p[1], p[2]);
+    const real d = sqrt(d2);
+    if (!(d <= r)) return;
+    ld3(vel[j], vj);
+    ld3(ab[j], aj);
+    const real h = real(2) * l0 / (d2 * d);
+    const real hp = real(-6) * l0 / (d2 * d2);
+    for (int t = 0; t < 3; t++) {
+      dv[t] = vj[t] - v[t];
+      da[t] = a[t] - aj[t];
+      vacc[t] -= kn * h * da[t];
+      S[t] += h * dv[t];
+    }
+    const real w = kn * hp / d * dotv<real, DIM>(da, dv);
+    for (int t = 0; t < 3; t++) xacc[t] -= w * p[t];
+  };
+  int nch = 0;
+  auto walk = [&]() {
+    walk_chunks(chunks, (size_t)n_alloc, k, nch, [&](bool, int j) { body(j); });
+    nch = 0;
+  };
   const int zlo = DIM == 3 ? q.iz - 1 : 0, zhi = DIM == 3 ? q.iz + 1 : 0;
+  const real px = x[0], py = x[1], pz = x[2], lim = c.r2_hi;
   for (int jz = zlo; jz <= zhi; jz++)
     for (int jy = q.iy - 1; jy <= q.iy + 1; jy++) {
       const int b = fcs[cell_key(q.ix - 1, jy, jz, c)], e = fcs[cell_key(q.ix + 1, jy, jz, c) + 1];
-      for_hits(
-          b, e,
-          [&](int j) {
-            const Vec<real, DIM> pj = pos[j];
-            return j != k && norm2<real, DIM>(pj.x - x[0], pj.y - x[1], vz(pj) - x[2]) <= c.r2_hi;
-          },
-          [&](int j) {
-            real y[3], p[3], vj[3], aj[3], dv[3], da[3];
-            ld3(pos[j], y);
-            for (int t = 0; t < 3; t++) p[t] = y[t] - x[t];
-            const real d2 = norm2<real, DIM>(p[0], p[1], p[2]);
-            const real d = sqrt(d2);
-            if (!(d <= r)) return;
-            ld3(vel[j], vj);
-            ld3(ab[j], aj);
-            const real h = real(2) * l0 / (d2 * d);
-            const real hp = real(-6) * l0 / (d2 * d2);
-            for (int t = 0; t < 3; t++) {
-              dv[t] = vj[t] - v[t];
-              da[t] = a[t] - aj[t];
-              vacc[t] -= kn * h * da[t];
-              S[t] += h * dv[t];
+      scan_row_staged(
+          stage, b, e, act, pos,
+          [&](const V& pj) { return near2(pj.x - px, pj.y - py, vz(pj) - pz) <= lim; },
+          [&](int base, unsigned long long m) {
+            if (unsigned(k - base) < 64u) m &= ~(1ull << (k - base));  // j != k
+            if (m) {
+              if (nch == kChunkCap) walk();
+              chunks[(size_t)nch * n_alloc + k] = Chunk{base, 0, m};
+              nch++;
             }
-            const real w = kn * hp / d * dotv<real, DIM>(da, dv);
-            for (int t = 0; t < 3; t++) xacc[t] -= w * p[t];
           });
     }
+  if (!act) return;
+  walk();
   if (mnear[cell_key(q.ix, q.iy, q.iz, c)])
     for_stencil<real, DIM>(q, c, mcs, [&](int m) {
       real y[3], p[3];

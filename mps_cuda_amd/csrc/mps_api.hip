@@ -956,6 +956,7 @@ struct Impl : ImplBase {
   real *nupart = nullptr, *cpart = nullptr;
   int *argm = nullptr, *inv = nullptr, *oflow = nullptr;
   double *acc2 = nullptr, *seed_x = nullptr, *seed_v = nullptr, *gout_x = nullptr, *gout_v = nullptr;
+  double* sum_part = nullptr;  // k_sum_parts partials
 
   template <class T>
   int aalloc(std::vector<void*>& list, T** p, size_t n) {
@@ -1098,6 +1099,7 @@ struct Impl : ImplBase {
         (rc = aalloc(adj_allocs, &nupart, nf)) || (rc = aalloc(adj_allocs, &cpart, nf)) ||
         (rc = aalloc(adj_allocs, &argm, nf)) || (rc = aalloc(adj_allocs, &inv, n_total)) ||
         (rc = aalloc(adj_allocs, &oflow, 1)) || (rc = aalloc(adj_allocs, &acc2, 2)) ||
+        (rc = aalloc(adj_allocs, &sum_part, mps::kSumParts)) ||
         (rc = aalloc(adj_allocs, &seed_x, n3)) || (rc = aalloc(adj_allocs, &seed_v, n3)) ||
         (rc = aalloc(adj_allocs, &gout_x, n3)) || (rc = aalloc(adj_allocs, &gout_v, n3)))
       return rc;
@@ -1143,9 +1145,8 @@ struct Impl : ImplBase {
     run_nlist();
     d2d(T.nlist, nlist, (size_t)mps::kNList * nf);
     d2d(T.ncount, ncount, nf);
-    run_col(true);
+    run_colprs();
     d2d(T.v2, fvel[0], nf);
-    run_prs();
     d2d(T.P, fprs[0], nf);
     d2d(T.nd, fnd[0], nf);
     mirror();
@@ -1157,6 +1158,14 @@ struct Impl : ImplBase {
     d2d(T.typ2, ftyp[0], nf);
     if (wall_pav && nw > 0)
       hipLaunchKernelGGL((mps::k_wall_pav<real>), dim3(nblk(nw)), B, 0, stream, wpav, wprs, nw);
+  }
+
+  // *acc += sum of a over the live entries of cs (deterministic order)
+  void sum_into(const real* a, const int* cs, double* acc) {
+    hipLaunchKernelGGL((mps::k_sum_parts<real>), dim3(mps::kSumParts), dim3(mps::kBlock), 0, stream, a, cs, tot,
+                       sum_part);
+    hipLaunchKernelGGL(mps::k_sum_final, dim3(1), dim3(mps::kSumParts), 0, stream, (const double*)sum_part,
+                       acc);
   }
 
   // Reverse of one step: (xbA, vbA) = adjoints of the step's output
@@ -1183,8 +1192,7 @@ struct Impl : ImplBase {
             (const real*)PBb, xbA, Pb);
     hipLaunchKernelGGL((mps::k_adj_prs_pre<real>), G, B, 0, stream, (const int*)T.fcs2, tot, nf,
                        (const real*)T.nd, (const real*)T.P, (const real*)Pb, c, hks[0].snd, nbar, cpart);
-    hipLaunchKernelGGL((mps::k_sum_into<real>), dim3(1), dim3(1024), 0, stream, (const real*)cpart,
-                       (const int*)T.fcs2, tot, acc2 + 1);
+    sum_into(cpart, T.fcs2, acc2 + 1);
     TLAUNCH("adj_prs", (mps::k_adj_prs<real, DIM>), G, B, 0, stream, (const V*)T.x1, (const int*)T.fcs2, nf,
             (const int*)T.nlist, (const int*)T.ncount, (const real*)nbar, (const V*)T.A.pos,
             (const int*)T.A.cs, (const uint8_t*)T.A.near, c, xbA, yAb);
@@ -1199,9 +1207,8 @@ struct Impl : ImplBase {
             abuf);
     TLAUNCH("adj_vsc", (mps::k_adj_vsc<real, DIM>), G, B, 0, stream, (const V*)T.x, (const V*)T.v,
             (const int*)T.fcs1, (const V*)T.A.pos, (const int*)T.A.cs, (const uint8_t*)T.A.near,
-            (const V*)abuf, c, real(1) / hks[0].n0, xbA, vbB, yAb, nupart);
-    hipLaunchKernelGGL((mps::k_sum_into<real>), dim3(1), dim3(1024), 0, stream, (const real*)nupart,
-                       (const int*)T.fcs1, tot, acc2);
+            (const V*)abuf, c, real(1) / hks[0].n0, chunks, nf, xbA, vbB, yAb, nupart);
+    sum_into(nupart, T.fcs1, acc2);
     hipLaunchKernelGGL((mps::k_adj_mirror_parent<real, DIM>), GM, B, 0, stream, (const V*)T.A.pos,
                        (const int*)(T.A.cs + tot), (const int*)T.A.par, (const V*)T.x, (const V*)yAb,
                        (const real*)nullptr, xbA, (real*)nullptr);
