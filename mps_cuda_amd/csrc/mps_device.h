@@ -168,11 +168,24 @@ struct Divisor<float> {
     const unsigned u = __float_as_uint(a) & 0x7fffffffu;
     return (u == 0u) | (u - 0x21800000u <= 0x3c000000u);
   }
+  // fits(a0) & fits(a1) & fits(a2) as one max3 and one min3 over the bit
+  // patterns of |a| (u - 1 maps a zero to the largest unsigned value).
+  __device__ static bool fit3(float a0, float a1, float a2) {
+    const unsigned u0 = __float_as_uint(a0) & 0x7fffffffu, u1 = __float_as_uint(a1) & 0x7fffffffu,
+                   u2 = __float_as_uint(a2) & 0x7fffffffu;
+    const unsigned hi = max(u0, max(u1, u2)), lo = min(u0 - 1u, min(u1 - 1u, u2 - 1u));
+    return (hi <= 0x5d800000u) & (lo >= 0x217fffffu);
+  }
   __device__ float recip() const { return y; }
-  __device__ float div(float a) const {
+  __device__ float div(float a) const { return __builtin_copysignf(div_sz(a), a); }
+  // div without the sign fix of a zero quotient: equal to a / b except that
+  // a zero quotient (a = 0) may carry either sign.  For terms added to a sum
+  // that starts at +0 this changes nothing: x + (+0) = x + (-0) for x != 0,
+  // and +0 + (-0) = +0 when rounding to nearest.
+  __device__ float div_sz(float a) const {
     const float q = a * y;
     const float r = __builtin_fmaf(-b, q, a);
-    return __builtin_copysignf(__builtin_fmaf(r, y, q), a);
+    return __builtin_fmaf(r, y, q);
   }
 };
 
@@ -182,8 +195,10 @@ struct Divisor<double> {
   bool ok = true;
   __device__ explicit Divisor(double b_) : b(b_) {}
   __device__ static bool fits(double) { return true; }
+  __device__ static bool fit3(double, double, double) { return true; }
   __device__ double recip() const { return 1.0 / b; }
   __device__ double div(double a) const { return a / b; }
+  __device__ double div_sz(double a) const { return a / b; }
 };
 
 // |q|^2 in the reference's operand order (x, then y, then z).

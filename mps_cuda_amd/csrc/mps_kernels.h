@@ -919,10 +919,10 @@ __global__ __launch_bounds__(kBlock) void k_vsc(
       const real nx = real(2) * (u.x - vx) * l0, ny = real(2) * (u.y - vy) * l0;
       const real nz = DIM == 3 ? real(2) * (vz(u) - vvz) * l0 : real(0);
       const Divisor<real> D(dist);
-      if (D.ok & D.fits(nx) & D.fits(ny) & D.fits(nz)) {
-        ax += D.div(D.div(D.div(nx)));
-        ay += D.div(D.div(D.div(ny)));
-        if constexpr (DIM == 3) az += D.div(D.div(D.div(nz)));
+      if (D.ok & D.fit3(nx, ny, nz)) {  // the sums start at +0: div_sz gives them what / gives
+        ax += D.div_sz(D.div_sz(D.div_sz(nx)));
+        ay += D.div_sz(D.div_sz(D.div_sz(ny)));
+        if constexpr (DIM == 3) az += D.div_sz(D.div_sz(D.div_sz(nz)));
       } else {
         ax += nx / dist / dist / dist;
         ay += ny / dist / dist / dist;
