@@ -63,7 +63,7 @@ KERNEL_SYMBOL = {  # timing name -> kernel symbol in the rocprofv3 summaries
     "col_prs": "mps::k_colprs_list<float, 3>",
     "prs": "mps::k_prs_list<float, 3>",
     "grad_up2": "mps::k_grad_list<float, 3, true>",
-    "mirror_gen": "mps::k_mirror_wave<float, 3>",
+    "mirror_gen": "mps::k_mirror_group<float, 3>",
     "nlist": "mps::k_nlist_build<float, 3>",
 }
 

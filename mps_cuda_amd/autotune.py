@@ -6,8 +6,8 @@ The tuned knobs change only how work is laid onto the GPU, never a result
   block      workgroup size of the per-particle kernels (64, 128, 256).
              Small scenes waste up to block-1 lanes each in scene-uniform
              workgroups, large ones prefer fewer, bigger workgroups.
-  wave_grid  workgroups of the one-wave-per-particle mirror kernel
-             (grid-stride over the near-wall particles).
+  wave_grid  workgroups of the mirror kernel (one wavefront each, four
+             near-wall particles at a time, grid-stride).
   graph      steps recorded per hipGraph replay (0 = eager launches).
 
 Search: coordinate descent from the defaults (block, then wave_grid, then

@@ -179,7 +179,7 @@ struct Impl : ImplBase {
   int* blk_off = nullptr;          // scene-uniform workgroup table [S+1]
   std::vector<int> nf_scene, nw_scene;  // fluid-set / wall-set entries per scene
   int block = mps::kBlock;         // workgroup size of the scene-uniform kernels
-  int wave_grid = 16384;           // workgroups of the mirror-wave kernel
+  int wave_grid = 16384;           // workgroups of the mirror kernel (grid-stride)
   int sb_grid = 1;                 // workgroups of the scene-uniform kernels
   int n_total = 0, nf = 0, nw = 0, mcap = 0, tot = 0;
   bool acc_live = false, wall_pav = false;
@@ -520,7 +520,7 @@ struct Impl : ImplBase {
     if (nf > 0 && use_cls) {
       TLAUNCH("mirror_near", (mps::k_mirror_near<real, DIM>), SG, SB, 0, stream, fpos[0],
               (const int*)fcs, sc(), (const uint8_t*)wnear, near_list, near_scene, mcount);
-      TLAUNCH("mirror_gen", (mps::k_mirror_wave<real, DIM>), dim3(std::max(1, std::min(nf, wave_grid))),
+      TLAUNCH("mirror_gen", (mps::k_mirror_group<real, DIM>), dim3(std::max(1, std::min(nf, wave_grid))),
               dim3(64), 0, stream, fpos[0], fprs[0], fid[0], near_list, near_scene, mcount,
               (const mps::Cst<real>*)dtab, wpos, wnrm, wcls, wcs, mraw_pos, mraw_prs, mraw_id,
               mraw_key);

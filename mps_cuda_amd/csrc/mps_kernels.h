@@ -515,10 +515,11 @@ __device__ inline int mirror_key(real mx, real my, real mz, const Cst<real>& c) 
 }
 
 // ---------------------------------------------------------------------------
-// Wave-parallel mirror generation (at most 64 wall-normal classes).
+// Group-parallel mirror generation (at most 64 wall-normal classes).
 //
 // k_mirror_near lists the live fluid particles whose stencil holds any wall.
-// k_mirror_wave then gives each listed particle one wavefront:
+// k_mirror_group then gives each listed particle a group of kMG lanes
+// (64 / kMG particles per wavefront):
 //  * the lanes split the wall candidates of the stencil; in the reference's
 //    scan order (z, y, x cells, then id) the candidates appear in increasing
 //    wall-array index, so "the first wall met of a normal within r"
@@ -528,8 +529,9 @@ __device__ inline int mirror_key(real mx, real my, real mz, const Cst<real>& c) 
 //    (kernel.cu:777-795) is the same over Edge walls.
 //  * each plane mirror and the edge mirror is culled by d_MRRinout's
 //    nearest-wall test (kernel.cu:944-983): the lanes split the candidates,
-//    keep the first minimum of their share, and a wave reduction picks the
-//    smallest (dist, index), which is the reference's first-met minimum.
+//    keep the lexicographic (dist, index) minimum of their share, and a group
+//    reduction picks the overall one, which is the reference's first-met
+//    minimum.
 //  * slot k of listed particle t goes to raw entry 10*t + k (key -1 if empty).
 
 constexpr int kInf = 0x7fffffff;
@@ -583,30 +585,53 @@ struct Rows {
   }
 };
 
-template <class real>
-struct BestWall {
-  real dist;
-  int j;
-};
+// A group of kMG lanes per particle keeps most lanes busy (a stencil row
+// holds ~30 walls) and puts several particles' chains of dependent loads in
+// flight per wavefront.
+constexpr int kMG = 16;
+// lane gl < kNumMRR of a group owns registry slot gl (clear, stage, write out)
+static_assert(kMG >= kNumMRR && (kMG & (kMG - 1)) == 0 && 64 % kMG == 0, "group width");
 
-// Wave-wide nearest wall within r of m (first met wins ties); j = -1 if none.
-// Lane l takes candidates b + l, b + l + 64, ... of every row; the first
-// candidate of each row is loaded for all rows at once.  Every lane keeps
-// the lexicographic (dist, index) minimum of its candidates, so its visiting
-// order does not matter, and the wave reduction gives the overall one: the
-// minimum distance met first in scan order (ascending wall index).
-template <class real, int DIM>
-__device__ inline BestWall<real> wave_nearest_wall(real mx, real my, real mz,
-                                                   const Vec<real, DIM>* __restrict__ wpos,
-                                                   const int* __restrict__ wcs, const Cst<real>& c,
-                                                   int lane) {
-  const Cell<real, DIM> q = cell_of<real, DIM>(mkvec<real, DIM>(mx, my, mz), c);
+// visit(j, wpos[j], wcls[j]) for every wall candidate of the stencil rows of
+// q; lane gl of the group takes candidates b + gl, b + gl + kMG, ... of every
+// row, one per row per round, all rows' loads of a round issued together.
+template <class real, int DIM, class Visit>
+__device__ inline void group_rows(const int* __restrict__ wcs, const Vec<real, DIM>* __restrict__ wpos,
+                                  const uint8_t* __restrict__ wcls, const Cell<real, DIM>& q,
+                                  const Cst<real>& c, int gl, bool act, Visit&& visit) {
+  if (!act) return;
   const Rows<real, DIM> R(wcs, q, c);
   constexpr int NR = Rows<real, DIM>::NR;
+  for (int off = 0;; off += kMG) {
+    Vec<real, DIM> w[NR];
+    int cl[NR];
+    bool more = false;
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+      const int j = R.b[r] + off + gl, jj = j < R.e[r] ? j : 0;  // 0: a valid wall index, not visited
+      w[r] = wpos[jj];
+      cl[r] = wcls ? int(wcls[jj]) : 0;
+      more = more || R.b[r] + off + kMG < R.e[r];
+    }
+#pragma unroll
+    for (int r = 0; r < NR; r++)
+      if (R.b[r] + off + gl < R.e[r]) visit(R.b[r] + off + gl, w[r], cl[r]);
+    if (!more) break;
+  }
+}
+
+// Nearest wall within r of m for a group (first met wins ties): every lane
+// keeps the lexicographic (dist, index) minimum of its candidates and the
+// group reduction picks the overall one.  True when m is culled.
+template <class real, int DIM>
+__device__ inline bool group_culled(real mx, real my, real mz, const Vec<real, DIM>* __restrict__ wpos,
+                                    const Vec<real, DIM>* __restrict__ wnrm, const int* __restrict__ wcs,
+                                    const Cst<real>& c, int gl) {
+  const Cell<real, DIM> q = cell_of<real, DIM>(mkvec<real, DIM>(mx, my, mz), c);
   real best = real(10) * c.r;
   real lim = c.r2_hi;
   int bj = -1;
-  auto visit = [&](int j, const Vec<real, DIM>& w) {
+  group_rows<real, DIM>(wcs, wpos, nullptr, q, c, gl, true, [&](int j, const Vec<real, DIM>& w, int) {
     const real d2 = norm2<real, DIM>(mx - w.x, my - w.y, mz - vz(w));
     if (!(d2 <= lim)) return;
     const real dist = sqrt(d2);
@@ -615,20 +640,8 @@ __device__ inline BestWall<real> wave_nearest_wall(real mx, real my, real mz,
       bj = j;
       lim = d2;
     }
-  };
-  Vec<real, DIM> w[NR];
-#pragma unroll
-  for (int r = 0; r < NR; r++) {
-    const int j = R.b[r] + lane;
-    w[r] = wpos[j < R.e[r] ? j : 0];
-  }
-#pragma unroll
-  for (int r = 0; r < NR; r++)
-    if (R.b[r] + lane < R.e[r]) visit(R.b[r] + lane, w[r]);
-  for (int r = 0; r < NR; r++)  // rows of more than 64 candidates
-    for (int j = R.b[r] + 64 + lane; j < R.e[r]; j += 64) visit(j, wpos[j]);
-  // lexicographic (dist, j) minimum over the wave; lanes without a hit hold j = -1
-  for (int off = 32; off >= 1; off >>= 1) {
+  });
+  for (int off = kMG / 2; off >= 1; off >>= 1) {
     const real od = __shfl_xor(best, off);
     const int oj = __shfl_xor(bj, off);
     const bool take = oj >= 0 && (bj < 0 || od < best || (od == best && oj < bj));
@@ -637,21 +650,13 @@ __device__ inline BestWall<real> wave_nearest_wall(real mx, real my, real mz,
       bj = oj;
     }
   }
-  return {best, bj};
-}
-
-template <class real, int DIM>
-__device__ inline bool wave_culled(real mx, real my, real mz, const Vec<real, DIM>* __restrict__ wpos,
-                                   const Vec<real, DIM>* __restrict__ wnrm,
-                                   const int* __restrict__ wcs, const Cst<real>& c, int lane) {
-  const BestWall<real> bw = wave_nearest_wall<real, DIM>(mx, my, mz, wpos, wcs, c, lane);
-  if (bw.j < 0) return false;
-  const Vec<real, DIM> w = wpos[bw.j], n = wnrm[bw.j];
+  if (bj < 0) return false;
+  const Vec<real, DIM> w = wpos[bj], n = wnrm[bj];
   return dot3<real, DIM>(mx - w.x, my - w.y, mz - vz(w), n.x, n.y, vz(n)) >= real(0);
 }
 
 template <class real, int DIM>
-__global__ __launch_bounds__(64) void k_mirror_wave(
+__global__ __launch_bounds__(64) void k_mirror_group(
     const Vec<real, DIM>* __restrict__ fpos, const real* __restrict__ fprs,
     const int* __restrict__ fid, const int* __restrict__ near_list,
     const int* __restrict__ near_scene, const int* __restrict__ near_count,
@@ -659,89 +664,95 @@ __global__ __launch_bounds__(64) void k_mirror_wave(
     const Vec<real, DIM>* __restrict__ wnrm, const uint8_t* __restrict__ wcls,
     const int* __restrict__ wcs, Vec<real, DIM>* __restrict__ mpos, real* __restrict__ mprs,
     int* __restrict__ mid, int* __restrict__ mkey) {
-  __shared__ int first_reg[64], first_edge[64], sorted_reg[kNumMRR], sorted_edge[kNumMRR];
-  __shared__ Vec<real, DIM> s_pos[kNumMRR];
-  __shared__ real s_prs[kNumMRR];
-  __shared__ int s_id[kNumMRR], s_key[kNumMRR];
-  const int lane = threadIdx.x;
+  constexpr int P = 64 / kMG;
+  __shared__ int first_reg[P][64], first_edge[P][64], sorted_reg[P][kNumMRR], sorted_edge[P][kNumMRR];
+  __shared__ Vec<real, DIM> s_pos[P][kNumMRR];
+  __shared__ real s_prs[P][kNumMRR];
+  __shared__ int s_id[P][kNumMRR], s_key[P][kNumMRR];
+  const int lane = threadIdx.x, g = lane / kMG, gl = lane % kMG;
   const int nnear = *near_count;
-  for (int t = blockIdx.x; t < nnear; t += gridDim.x) {
-    const int i = near_list[t];
-    const Cst<real> c = tab[near_scene[t]];
+  for (int t0 = blockIdx.x * P; t0 < nnear; t0 += gridDim.x * P) {
+    const int t = t0 + g;
+    const bool act = t < nnear;
+    const int tt = act ? t : t0;
+    const int i = near_list[tt];
+    const Cst<real> c = tab[near_scene[tt]];
     const Vec<real, DIM> p = fpos[i];
     const real px = p.x, py = p.y, pz = vz(p);
     const Cell<real, DIM> q = cell_of<real, DIM>(p, c);
-    first_reg[lane] = kInf;
-    first_edge[lane] = kInf;
-    if (lane < kNumMRR) s_key[lane] = -1;
+    for (int k = gl; k < 64; k += kMG) {
+      first_reg[g][k] = kInf;
+      first_edge[g][k] = kInf;
+    }
+    if (gl < kNumMRR) s_key[g][gl] = -1;
     __syncthreads();
     const real r2_2 = real(1) * c.r2;
-    {
-      const Rows<real, DIM> R(wcs, q, c);
-      constexpr int NR = Rows<real, DIM>::NR;
-      auto visit = [&](int j, const Vec<real, DIM>& w, int cl) {
-        if (norm2<real, DIM>(px - w.x, py - w.y, pz - vz(w)) <= r2_2) {
-          atomicMin(&first_reg[cl & 63], j);
-          if (cl & 0x80) atomicMin(&first_edge[cl & 63], j);
-        }
-      };
-      Vec<real, DIM> w[NR];
-      int cl[NR];
-#pragma unroll
-      for (int r = 0; r < NR; r++) {
-        const int j = R.b[r] + lane, jj = j < R.e[r] ? j : 0;
-        w[r] = wpos[jj];
-        cl[r] = wcls[jj];
+    group_rows<real, DIM>(wcs, wpos, wcls, q, c, gl, act, [&](int j, const Vec<real, DIM>& w, int cl) {
+      if (norm2<real, DIM>(px - w.x, py - w.y, pz - vz(w)) <= r2_2) {
+        atomicMin(&first_reg[g][cl & 63], j);
+        if (cl & 0x80) atomicMin(&first_edge[g][cl & 63], j);
       }
-#pragma unroll
-      for (int r = 0; r < NR; r++)
-        if (R.b[r] + lane < R.e[r]) visit(R.b[r] + lane, w[r], cl[r]);
-      for (int r = 0; r < NR; r++)  // rows of more than 64 candidates
-        for (int j = R.b[r] + 64 + lane; j < R.e[r]; j += 64) visit(j, wpos[j], wcls[j]);
-    }
+    });
     __syncthreads();
-    // registry order: rank of each class's first index among the classes hit
-    const int fr = first_reg[lane], fe = first_edge[lane];
-    const unsigned long long hit = __ballot(fr != kInf), hite = __ballot(fe != kInf);
-    int rr = 0, re_ = 0;
-    for (unsigned long long m = hit; m; m &= m - 1) rr += first_reg[__ffsll((long long)m) - 1] < fr;
-    for (unsigned long long m = hite; m; m &= m - 1) re_ += first_edge[__ffsll((long long)m) - 1] < fe;
-    if (fr != kInf && rr < kNumMRR) sorted_reg[rr] = fr;
-    if (fe != kInf && re_ < kNumMRR) sorted_edge[re_] = fe;
-    const int nreg = min(__popcll(hit), kNumMRR);
-    const int ne = min(__popcll(hite), kNumMRR);
+    // registry order: rank of each hit class's first index among the group's hit classes
+    unsigned long long hit = 0, hite = 0;
+    for (int m = 0; m < 64 / kMG; m++) {
+      const int k = gl + kMG * m;
+      if (first_reg[g][k] != kInf) hit |= 1ull << k;
+      if (first_edge[g][k] != kInf) hite |= 1ull << k;
+    }
+    for (int off = kMG / 2; off >= 1; off >>= 1) {
+      hit |= __shfl_xor(hit, off);
+      hite |= __shfl_xor(hite, off);
+    }
+    for (int m = 0; m < 64 / kMG; m++) {
+      const int k = gl + kMG * m;
+      const int fr = first_reg[g][k], fe = first_edge[g][k];
+      if (fr != kInf) {
+        int rr = 0;
+        for (unsigned long long b = hit; b; b &= b - 1) rr += first_reg[g][__ffsll((long long)b) - 1] < fr;
+        if (rr < kNumMRR) sorted_reg[g][rr] = fr;
+      }
+      if (fe != kInf) {
+        int re_ = 0;
+        for (unsigned long long b = hite; b; b &= b - 1) re_ += first_edge[g][__ffsll((long long)b) - 1] < fe;
+        if (re_ < kNumMRR) sorted_edge[g][re_] = fe;
+      }
+    }
+    const int nreg = act ? min(__popcll(hit), kNumMRR) : 0;
+    const int ne = act ? min(__popcll(hite), kNumMRR) : 0;
     __syncthreads();
     const int base_id = fid[i] * kNumMRR;
     const real prs_i = fprs[i];
     for (int k = 0; k < nreg; k++) {  // plane mirrors, kernel.cu:811-838
-      const int F = sorted_reg[k];
+      const int F = sorted_reg[g][k];
       real mx, my, mz;
       if (!plane_mirror<real, DIM>(px, py, pz, wpos[F], wnrm[F], mx, my, mz)) continue;
-      if (!wave_culled<real, DIM>(mx, my, mz, wpos, wnrm, wcs, c, lane) && lane == 0) {
-        s_pos[k] = mkvec<real, DIM>(mx, my, mz);
-        s_prs[k] = prs_i;
-        s_id[k] = base_id + k;
-        s_key[k] = mirror_key<real, DIM>(mx, my, mz, c);
+      if (!group_culled<real, DIM>(mx, my, mz, wpos, wnrm, wcs, c, gl) && gl == 0) {
+        s_pos[g][k] = mkvec<real, DIM>(mx, my, mz);
+        s_prs[g][k] = prs_i;
+        s_id[g][k] = base_id + k;
+        s_key[g][k] = mirror_key<real, DIM>(mx, my, mz, c);
       }
     }
     if (ne >= 2 && nreg < kNumMRR) {  // edge mirror, kernel.cu:840-888
       real mx, my, mz;
-      if (edge_mirror<real, DIM>(px, py, pz, sorted_edge, ne, wpos, wnrm, mx, my, mz) &&
-          !wave_culled<real, DIM>(mx, my, mz, wpos, wnrm, wcs, c, lane) && lane == 0) {
-        s_pos[nreg] = mkvec<real, DIM>(mx, my, mz);
-        s_prs[nreg] = prs_i;
-        s_id[nreg] = base_id + nreg;
-        s_key[nreg] = mirror_key<real, DIM>(mx, my, mz, c);
+      if (edge_mirror<real, DIM>(px, py, pz, sorted_edge[g], ne, wpos, wnrm, mx, my, mz) &&
+          !group_culled<real, DIM>(mx, my, mz, wpos, wnrm, wcs, c, gl) && gl == 0) {
+        s_pos[g][nreg] = mkvec<real, DIM>(mx, my, mz);
+        s_prs[g][nreg] = prs_i;
+        s_id[g][nreg] = base_id + nreg;
+        s_key[g][nreg] = mirror_key<real, DIM>(mx, my, mz, c);
       }
     }
     __syncthreads();
-    if (lane < kNumMRR) {
-      const size_t o = (size_t)t * kNumMRR + lane;
-      mkey[o] = s_key[lane];
-      if (s_key[lane] >= 0) {
-        mpos[o] = s_pos[lane];
-        mprs[o] = s_prs[lane];
-        mid[o] = s_id[lane];
+    if (act && gl < kNumMRR) {
+      const size_t o = (size_t)t * kNumMRR + gl;
+      mkey[o] = s_key[g][gl];
+      if (s_key[g][gl] >= 0) {
+        mpos[o] = s_pos[g][gl];
+        mprs[o] = s_prs[g][gl];
+        mid[o] = s_id[g][gl];
       }
     }
     __syncthreads();
