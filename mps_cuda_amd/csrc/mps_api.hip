@@ -210,8 +210,7 @@ struct Impl : ImplBase {
   uint8_t *wnear = nullptr, *mnear = nullptr;       // any wall / mirror in a bucket's stencil
   // bucket scratch (shared by the fluid and mirror builds)
   int *key = nullptr, *slot = nullptr, *tmp = nullptr, *tmp_id = nullptr, *perm = nullptr, *cnt = nullptr;
-  // scan levels: parts[l] holds block totals, parts_ex[l] their exclusive scan
-  std::vector<int*> parts, parts_ex;
+  int* part = nullptr;  // tile totals of the scan
   // captured step graph
   int graph_steps = 0;
   hipGraph_t graph = nullptr;
@@ -357,15 +356,7 @@ struct Impl : ImplBase {
         (rc = alloc(&tmp_id, scr)) ||
         (rc = alloc(&perm, scr)) || (rc = alloc(&cnt, (size_t)tot + 1)))
       return rc;
-    for (long long n = tot + 1;;) {  // scan levels for n = tot + 1
-      const long long nb = (n + 1 + mps::kScanTile - 1) / mps::kScanTile;
-      if (nb <= 1) break;
-      int *a, *b;
-      if ((rc = alloc(&a, nb)) || (rc = alloc(&b, nb + 1))) return rc;
-      parts.push_back(a);
-      parts_ex.push_back(b);
-      n = nb;
-    }
+    if ((rc = alloc(&part, ((size_t)tot + 2 + mps::kScanTile - 1) / mps::kScanTile))) return rc;
     HIP_TRY(hipMemcpy(dtab, csts.data(), S * sizeof(mps::Cst<real>), hipMemcpyHostToDevice));
 
     auto src = [&](int g, const double* mps_scene::*field, int k) {
@@ -446,6 +437,8 @@ struct Impl : ImplBase {
                        (const mps::Cst<real>*)dtab, S, tot, wnear);
     hipLaunchKernelGGL(mps::k_fill_int, dim3(nblk(tot + 2, 4096)), B, 0, stream, mcs, tot + 2, 0);
     hipLaunchKernelGGL(mps::k_fill_int, dim3(1), B, 0, stream, mcount, 1, 0);
+    // the counting array stays zero between builds (the scan clears it)
+    hipLaunchKernelGGL(mps::k_fill_int, dim3(nblk(tot + 1, 4096)), B, 0, stream, cnt, tot + 1, 0);
     hipLaunchKernelGGL((mps::k_fill<uint8_t>), dim3(nblk(tot, 4096)), B, 0, stream, mnear, tot,
                        (uint8_t)0);
     if ((rc = bucket())) return rc;  // ClcMPS starts with MkBkt (kernel.cu:1878)
@@ -453,18 +446,15 @@ struct Impl : ImplBase {
     return check_launch();
   }
 
-  // exclusive scan of in[0..n) into out[0..n] (out[n] = total)
-  void scan(const int* in, int n, int* out, int level) {
+  // exclusive scan of in[0..n) into out[0..n] (out[n] = total); in is
+  // zeroed as it is read (in = cnt, clear for the next count)
+  void scan(int* in, int n, int* out) {
     const int nb = (n + 1 + mps::kScanTile - 1) / mps::kScanTile;
-    if (nb <= 1) {
-      hipLaunchKernelGGL(mps::k_scan_down, dim3(1), dim3(mps::kBlock), 0, stream, in, n,
-                         (const int*)nullptr, out);
-      return;
-    }
-    hipLaunchKernelGGL(mps::k_scan_reduce, dim3(nb), dim3(mps::kBlock), 0, stream, in, n, parts[level]);
-    scan(parts[level], nb, parts_ex[level], level + 1);
+    if (nb > 1)
+      hipLaunchKernelGGL(mps::k_scan_reduce, dim3(nb), dim3(mps::kBlock), 0, stream, (const int*)in, n,
+                         part);
     hipLaunchKernelGGL(mps::k_scan_down, dim3(nb), dim3(mps::kBlock), 0, stream, in, n,
-                       (const int*)parts_ex[level], out);
+                       nb > 1 ? (const int*)part : nullptr, out);
   }
 
   mps::FluidArrays<real, DIM> fa(int b) {
@@ -492,30 +482,30 @@ struct Impl : ImplBase {
       hipLaunchKernelGGL(mps::k_fill_int, dim3(1), B, 0, stream, blk_off, S + 1, 0);
       return MPS_OK;
     }
-    hipLaunchKernelGGL(mps::k_fill_int, dim3(nblk(tot + 1, 4096)), B, 0, stream, cnt, tot + 1, 0);
     TLAUNCH("bucket_key", (mps::k_bkt_key<real, DIM>), dim3(nblk(nf)), B, 0, stream, fpos[0],
             (const int8_t*)ftyp[0], (const int*)fsid[0], (const mps::Cst<real>*)dtab, nf, tot, key,
             slot, cnt);
     tick("bucket_scan");
-    scan(cnt, tot + 1, fcs, 0);
+    scan(cnt, tot + 1, fcs);
     tock();
     TLAUNCH("bucket_scatter", mps::k_bkt_scatter, dim3(nblk(nf)), B, 0, stream, key, slot, fid[0], fcs,
             nf, (const int*)nullptr, 1, tmp, tmp_id);
     TLAUNCH("bucket_rank", mps::k_bkt_rank, dim3(nblk(nf)), B, 0, stream, tmp, tmp_id, key, fcs, nf,
-            (const int*)nullptr, tot, perm);
+            (const int*)nullptr, tot, perm, (int*)nullptr);
     TLAUNCH("bucket_gather", (mps::k_gather_fluid<real, DIM>), dim3(nblk(nf)), B, 0, stream, perm, nf,
             fa(0), fa(1));
     swap_all();
-    hipLaunchKernelGGL((mps::k_scene_blocks<real>), dim3(1), B, 0, stream, (const mps::Cst<real>*)dtab,
-                       S, (const int*)fcs, block, blk_off);
+    if (S > 1)
+      hipLaunchKernelGGL((mps::k_scene_blocks<real>), dim3(1), B, 0, stream, (const mps::Cst<real>*)dtab,
+                         S, (const int*)fcs, block, blk_off);
     return MPS_OK;
   }
 
   // ResetMRR + GenMRR_nonslip + MkBkt_MRR (kernel.cu:1906-1908).
   int mirror() {
     const dim3 B(mps::kBlock), SB(block), SG(sb_grid);
-    hipLaunchKernelGGL(mps::k_fill_int, dim3(1), B, 0, stream, mcount, 1, 0);
-    // raw slots: 10 per listed particle (wave path) or compacted (general path)
+    // mcount is zero here: the previous build's rank kernel cleared it
+    // raw slots: 10 per listed particle (group path) or compacted (general path)
     int mult = 1;
     if (nf > 0 && use_cls) {
       TLAUNCH("mirror_near", (mps::k_mirror_near<real, DIM>), SG, SB, 0, stream, fpos[0],
@@ -531,21 +521,19 @@ struct Impl : ImplBase {
               mcount, mcap);
     }
     const int g = nblk(mcap, 2048);
-    hipLaunchKernelGGL(mps::k_fill_int, dim3(nblk(tot + 1, 4096)), B, 0, stream, cnt, tot + 1, 0);
     TLAUNCH("mirror_key", mps::k_key_count, dim3(g), B, 0, stream, (const int*)mraw_key,
             (const int*)mcount, mult, slot, cnt);
     tick("mirror_scan");
-    scan(cnt, tot + 1, mcs, 0);
+    scan(cnt, tot + 1, mcs);
     tock();
     TLAUNCH("mirror_scatter", mps::k_bkt_scatter, dim3(g), B, 0, stream, (const int*)mraw_key, slot,
             mraw_id, mcs, 0, (const int*)mcount, mult, tmp, tmp_id);
     // live mirrors: mcs[tot]
     TLAUNCH("mirror_rank", mps::k_bkt_rank, dim3(g), B, 0, stream, tmp, tmp_id, (const int*)mraw_key, mcs,
-            0, (const int*)(mcs + tot), tot, perm);
+            0, (const int*)(mcs + tot), tot, perm, mcount);
     TLAUNCH("mirror_gather", (mps::k_gather_mirror<real, DIM>), dim3(g), B, 0, stream, perm,
-            (const int*)(mcs + tot), mraw_pos, mraw_prs, mraw_id, mpos, mprs, mid);
-    TLAUNCH("mirror_flags", (mps::k_cell_near<real, DIM>), dim3(nblk(tot)), B, 0, stream,
-            (const int*)mcs, (const mps::Cst<real>*)dtab, S, tot, mnear);
+            (const int*)(mcs + tot), mraw_pos, mraw_prs, mraw_id, mpos, mprs, mid, (const int*)mcs,
+            (const mps::Cst<real>*)dtab, S, tot, mnear);
     return MPS_OK;
   }
 
@@ -714,8 +702,9 @@ struct Impl : ImplBase {
     block = b;
     wave_grid = wg;
     update_grid();
-    hipLaunchKernelGGL((mps::k_scene_blocks<real>), dim3(1), dim3(mps::kBlock), 0, stream,
-                       (const mps::Cst<real>*)dtab, S, (const int*)fcs, block, blk_off);
+    if (S > 1)
+      hipLaunchKernelGGL((mps::k_scene_blocks<real>), dim3(1), dim3(mps::kBlock), 0, stream,
+                         (const mps::Cst<real>*)dtab, S, (const int*)fcs, block, blk_off);
     return check_launch();
   }
 

@@ -69,6 +69,7 @@ struct Cst {
 // Scene lookup for kernels launched with scene-uniform workgroups: block b
 // serves scene s when blk_off[s] <= b < blk_off[s+1], and covers live
 // fluid entries cs[cbase_s] + (b - blk_off[s]) * blockDim.x + threadIdx.x.
+// A single-scene handle does not use blk_off.
 template <class real>
 struct Scenes {
   const Cst<real>* tab;   // [S]
@@ -113,6 +114,13 @@ template <class real>
 __device__ inline bool scene_block(const Scenes<real>& sc, const int* __restrict__ cs, int& s,
                                    Cst<real>& c, int& first, int& end) {
   const int b = blockIdx.x;
+  if (sc.S == 1) {  // one scene: no table needed
+    s = 0;
+    c = sc.tab[0];
+    first = cs[c.cbase] + b * blockDim.x;
+    end = cs[c.cbase + c.ncell];
+    return first < end;
+  }
   if (b >= sc.blk_off[sc.S]) return false;
   int lo = 0, hi = sc.S;
   while (hi - lo > 1) {

@@ -171,19 +171,30 @@ __global__ __launch_bounds__(kBlock) void k_scan_reduce(const int* in, int n, in
   if (threadIdx.x == 0) part[blockIdx.x] = total;
 }
 
-// out[i] = sum(in[0..i)) + offset of the tile; out[n] = grand total.
-__global__ __launch_bounds__(kBlock) void k_scan_down(const int* in, int n, const int* part_excl,
-                                                     int* out) {
+// out[i] = sum(in[0..i)); out[n] = grand total.  The offset of a tile is the
+// sum of the totals of the tiles before it (part, from k_scan_reduce; null
+// for a single tile), added up by each workgroup itself.  The counts are
+// zeroed once read, so the counting array is clear for the next build.
+__global__ __launch_bounds__(kBlock) void k_scan_down(int* in, int n, const int* part, int* out) {
   __shared__ int sh[kBlock];
+  int o = 0;
+  if (part)
+    for (int k = threadIdx.x; k < int(blockIdx.x); k += kBlock) o += part[k];
+  int offset;
+  block_exclusive_scan(o, sh, offset);
   const int base = blockIdx.x * kScanTile + threadIdx.x * kScanIPT;
   int v[kScanIPT];
   int s = 0;
   for (int k = 0; k < kScanIPT; k++) {
-    v[k] = base + k < n ? in[base + k] : 0;
+    v[k] = 0;
+    if (base + k < n) {
+      v[k] = in[base + k];
+      in[base + k] = 0;
+    }
     s += v[k];
   }
   int total;
-  int run = block_exclusive_scan(s, sh, total) + (part_excl ? part_excl[blockIdx.x] : 0);
+  int run = block_exclusive_scan(s, sh, total) + offset;
   for (int k = 0; k < kScanIPT; k++) {
     if (base + k < n) out[base + k] = run;
     run += v[k];
@@ -231,26 +242,29 @@ __device__ inline int scene_of_cell(const Cst<real>* __restrict__ tab, int S, in
 // 27-bucket (9 in 2-D) stencil of bucket k.  Lets a particle skip a stencil
 // pass over a set that is empty around it.
 template <class real, int DIM>
-__global__ void k_cell_near(const int* __restrict__ cs, const Cst<real>* __restrict__ tab, int S,
-                            int tot, uint8_t* __restrict__ near) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= tot) return;
+__device__ inline uint8_t cell_near(const int* __restrict__ cs, const Cst<real>* __restrict__ tab, int S,
+                                    int k) {
   const Cst<real> c = tab[scene_of_cell(tab, S, k)];
   const int l = k - c.cbase;
   const int iz = DIM == 3 ? l / c.nbxy : 0;
   const int rem = l - iz * c.nbxy;
   const int iy = rem / c.nb[0], ix = rem - iy * c.nb[0];
   if (ix < 1 || ix > c.nb[0] - 2 || iy < 1 || iy > c.nb[1] - 2 ||
-      (DIM == 3 && (iz < 1 || iz > c.nb[2] - 2))) {
-    near[k] = 0;  // no particle is ever assigned to a border bucket
-    return;
-  }
+      (DIM == 3 && (iz < 1 || iz > c.nb[2] - 2)))
+    return 0;  // no particle is ever assigned to a border bucket
   bool any = false;
   const int zlo = DIM == 3 ? iz - 1 : 0, zhi = DIM == 3 ? iz + 1 : 0;
   for (int jz = zlo; jz <= zhi; jz++)
     for (int jy = iy - 1; jy <= iy + 1; jy++)
       any = any || cs[cell_key(ix - 1, jy, jz, c)] != cs[cell_key(ix + 1, jy, jz, c) + 1];
-  near[k] = any ? 1 : 0;
+  return any ? 1 : 0;
+}
+
+template <class real, int DIM>
+__global__ void k_cell_near(const int* __restrict__ cs, const Cst<real>* __restrict__ tab, int S,
+                            int tot, uint8_t* __restrict__ near) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < tot) near[k] = cell_near<real, DIM>(cs, tab, S, k);
 }
 
 // ---------------------------------------------------------------------------
@@ -326,9 +340,12 @@ __global__ void k_bkt_scatter(const int* __restrict__ key, const int* __restrict
 // from tmp_id; past the cell's end the first id is read and not counted).
 // The deleted tail (key == tot) keeps the scatter order: its order is never
 // observed.
+// `clear` (may be null), no longer read once the scatter has run, is zeroed.
 __global__ void k_bkt_rank(const int* __restrict__ tmp, const int* __restrict__ tmp_id,
                            const int* __restrict__ key, const int* __restrict__ cs, int n_static,
-                           const int* __restrict__ n_dyn, int tot, int* __restrict__ perm) {
+                           const int* __restrict__ n_dyn, int tot, int* __restrict__ perm,
+                           int* __restrict__ clear) {
+  if (clear && blockIdx.x == 0 && threadIdx.x == 0) *clear = 0;
   const int n = n_dyn ? *n_dyn : n_static;
   for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
     const int i = tmp[s];
@@ -381,12 +398,16 @@ __global__ void k_gather_fluid(const int* __restrict__ perm, int n, FluidArrays<
   }
 }
 
+// Also sets near[k] (any mirror in the stencil of bucket k) from the new
+// cell starts, as k_cell_near does.
 template <class real, int DIM>
 __global__ void k_gather_mirror(const int* __restrict__ perm, const int* __restrict__ n_dyn,
                                 const Vec<real, DIM>* __restrict__ spos,
                                 const real* __restrict__ sprs, const int* __restrict__ sid,
                                 Vec<real, DIM>* __restrict__ dpos, real* __restrict__ dprs,
-                                int* __restrict__ did) {
+                                int* __restrict__ did, const int* __restrict__ cs,
+                                const Cst<real>* __restrict__ tab, int S, int tot,
+                                uint8_t* __restrict__ near) {
   const int n = *n_dyn;
   for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
     const int i = perm[s];
@@ -394,6 +415,8 @@ __global__ void k_gather_mirror(const int* __restrict__ perm, const int* __restr
     dprs[s] = sprs[i];
     did[s] = sid[i];
   }
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < tot; k += gridDim.x * blockDim.x)
+    near[k] = cell_near<real, DIM>(cs, tab, S, k);
 }
 
 // Copies the deleted tail [cs[tot], n) of double-buffered fields (fused
