@@ -214,7 +214,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_grad_pre(
     const int* __restrict__ mpar, const Vec<real, DIM>* __restrict__ ab, Cst<real> c,
     int* __restrict__ argmin, real* __restrict__ sg, real* __restrict__ pmin_out,
     Vec<real, DIM>* __restrict__ xb, Vec<real, DIM>* __restrict__ zb, real* __restrict__ pbm) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (i >= fcs[c.tot]) return;
   real x[3], a[3];
   ld3(pos[i], x);
@@ -283,7 +283,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_grad(
     int nf, const int* __restrict__ list, const int* __restrict__ count,
     const Vec<real, DIM>* __restrict__ ab, const int* __restrict__ argmin, const real* __restrict__ sg,
     const real* __restrict__ pmin, Cst<real> c, Vec<real, DIM>* __restrict__ xb, real* __restrict__ pb) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (k >= fcs[c.tot]) return;
   real x[3], a[3];
   ld3(pos[k], x);
@@ -375,7 +375,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_prs(
     const int* __restrict__ list, const int* __restrict__ count, const real* __restrict__ nbar,
     const Vec<real, DIM>* __restrict__ mpos, const int* __restrict__ mcs, const uint8_t* __restrict__ mnear,
     Cst<real> c, Vec<real, DIM>* __restrict__ xb, Vec<real, DIM>* __restrict__ yb) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (k >= fcs[c.tot]) return;
   real x[3];
   ld3(pos[k], x);
@@ -432,7 +432,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_col(
     const uint8_t* __restrict__ wnear, const uint8_t* __restrict__ mnear,
     const Vec<real, DIM>* __restrict__ gb, Cst<real> c, Vec<real, DIM>* __restrict__ xb,
     Vec<real, DIM>* __restrict__ v1b, Vec<real, DIM>* __restrict__ yb) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (k >= fcs[c.tot]) return;
   real x[3], v[3], g[3];
   ld3(pos[k], x);
@@ -564,10 +564,11 @@ __global__ __launch_bounds__(kBlock) void k_adj_vsc(
   using V = Vec<real, DIM>;
   __shared__ V s_stage[kBlock / 64][stage_cap<V>() + 8];
   const int n_live = fcs[c.tot];
-  if (int(blockIdx.x * blockDim.x) >= n_live) return;  // whole workgroup
+  const int b0 = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x;
+  if (b0 >= n_live) return;  // whole workgroup
   // every lane stays for the wave-wide staging; act marks a particle
-  const bool act = int(blockIdx.x * blockDim.x + threadIdx.x) < n_live;
-  const int k = act ? int(blockIdx.x * blockDim.x + threadIdx.x) : 0;
+  const bool act = b0 + int(threadIdx.x) < n_live;
+  const int k = act ? b0 + int(threadIdx.x) : 0;
   V* stage = s_stage[threadIdx.x / 64];
   real x[3], v[3], a[3];
   ld3(pos[k], x);

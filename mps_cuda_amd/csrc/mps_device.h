@@ -107,13 +107,25 @@ __host__ __device__ inline int cell_key(int ix, int iy, int iz, const Cst<real>&
   return c.cbase + iz * c.nbxy + iy * c.nb[0] + ix;
 }
 
+// Workgroups are dispatched to the 8 XCDs of the chip in turn (workgroup w
+// to XCD w mod 8), and each XCD has its own L2.  xcd_tile maps w to a tile
+// index so that each XCD gets one contiguous range of tiles: neighbouring
+// tiles of a sorted particle set, which read the same neighbours, then share
+// an L2.  Any bijection gives the same results; this one only changes where
+// the work runs.
+constexpr int kXCD = 8;
+__device__ inline int xcd_tile(int w, int n) {
+  const int x = w % kXCD, k = w / kXCD, q = n / kXCD, r = n % kXCD;
+  return x * q + min(x, r) + k;
+}
+
 // Scene of workgroup blockIdx.x (uniform: kept in scalar registers), and the
 // live range [begin, end) of that scene's fluid entries.  Returns false for
 // surplus workgroups.
 template <class real>
 __device__ inline bool scene_block(const Scenes<real>& sc, const int* __restrict__ cs, int& s,
                                    Cst<real>& c, int& first, int& end) {
-  const int b = blockIdx.x;
+  const int b = xcd_tile(blockIdx.x, gridDim.x);
   if (sc.S == 1) {  // one scene: no table needed
     s = 0;
     c = sc.tab[0];
@@ -188,8 +200,9 @@ struct Divisor<float> {
   __device__ float div(float a) const { return __builtin_copysignf(div_sz(a), a); }
   // div without the sign fix of a zero quotient: equal to a / b except that
   // a zero quotient (a = 0) may carry either sign.  For terms added to a sum
-  // that starts at +0 this changes nothing: x + (+0) = x + (-0) for x != 0,
-  // and +0 + (-0) = +0 when rounding to nearest.
+  // that starts at +0 this changes nothing: such a sum is never -0 (an exact
+  // zero sum is +0 when rounding to nearest), and x + (+0) = x + (-0) for
+  // every x other than -0.
   __device__ float div_sz(float a) const {
     const float q = a * y;
     const float r = __builtin_fmaf(-b, q, a);
