@@ -291,6 +291,12 @@ int mps_compute_constants(const mps_params* params, const mps_scene* scene, mps_
 /* Surface_Edge once on the host (kernel.cu:553-686): edge[n] = WLLSE. */
 int mps_compute_edge(const mps_params* params, const mps_scene* scene, int8_t* edge);
 
+/* The handle's WLLSE in creation order (edge[n_total]; fluid entries are
+ * MPS_SURFACE): what MPS::Surface_Edge leaves in WLLSE after its copy to the
+ * host (kernel.cu:670-686).  mps_create computes it on the GPU
+ * (d_Surface_Edge, kernel.cu:553-668) unless the scene passes `edge`. */
+int mps_get_edge(mps_handle* h, int8_t* edge);
+
 /* ---- diagnostics --------------------------------------------------------- */
 
 /* Runs the device's sqrt(a) and a/b on n values (precision MPS_FP32 rounds

@@ -63,6 +63,7 @@ struct ImplBase {
   virtual int adj_backward(const double* xbar, const double* vbar, double* gx0, double* gv0,
                            double* gp) = 0;
   virtual void adj_release() = 0;
+  virtual int get_edge(int8_t* edge) = 0;
   hipStream_t stream = nullptr;
   hipStream_t own_stream = nullptr;
   std::vector<void*> allocs;
@@ -296,10 +297,16 @@ struct Impl : ImplBase {
     std::vector<int> wkey_all(n_total, 0), scene_of(n_total);
     nf_scene.assign(S, 0);
     nw_scene.assign(S, 0);
+    // Surface_Edge: from the caller, else on the GPU after the walls are
+    // uploaded (k_surface_edge), else on the host (mps_host.cpp)
+    std::vector<uint8_t> edge_gpu(S, 0);
     for (int s = 0; s < S; s++) {
       std::vector<int8_t> e;
       if (ss[s].edge) e.assign(ss[s].edge, ss[s].edge + ss[s].n);
-      else mps::compute_edge<real>(ps[s], ss[s], e);
+      else if (edge_on_gpu(ps[s], ss[s], s)) {
+        edge_gpu[s] = 1;
+        e.assign(ss[s].n, int8_t(MPS_SURFACE));
+      } else mps::compute_edge<real>(ps[s], ss[s], e);
       for (int i = 0; i < ss[s].n; i++) {
         const int g = poff[s] + i;
         typ_all[g] = ss[s].typ[i];
@@ -407,6 +414,18 @@ struct Impl : ImplBase {
       HIP_TRY(hipMemcpy(wprs, hprs.data(), nw * sizeof(real), hipMemcpyHostToDevice));
       HIP_TRY(hipMemcpy(wpav, hpav.data(), nw * sizeof(real), hipMemcpyHostToDevice));
       HIP_TRY(hipMemcpy(wcs, hcs.data(), (tot + 1) * sizeof(int), hipMemcpyHostToDevice));
+      if (std::any_of(edge_gpu.begin(), edge_gpu.end(), [](uint8_t f) { return f != 0; })) {
+        std::vector<double> p64(3 * (size_t)nw), R2s(S);
+        std::vector<int> wsid(nw);
+        for (int w = 0; w < nw; w++) {
+          const int g = host_wall_ids[w], sc_ = scene_of[g], i = g - poff[sc_];
+          for (int a = 0; a < 3; a++)
+            p64[a * (size_t)nw + w] = a < DIM ? ss[sc_].pos[a * (size_t)ss[sc_].n + i] : 0.0;
+          wsid[w] = sc_;
+        }
+        for (int s2 = 0; s2 < S; s2++) R2s[s2] = (ps[s2].edge_factor * ps[s2].l0) * (ps[s2].edge_factor * ps[s2].l0);
+        if ((rc = surface_edge_gpu(p64, wsid, R2s, edge_gpu))) return rc;
+      }
     }
     {  // the fluid set in creation order; the first bucket build sorts it
       std::vector<V> hp(nf), hv(nf), ha(nf, mps::mkvec<real, DIM>(0, 0, 0));
@@ -444,6 +463,68 @@ struct Impl : ImplBase {
     if ((rc = bucket())) return rc;  // ClcMPS starts with MkBkt (kernel.cu:1878)
     HIP_TRY(hipStreamSynchronize(stream));
     return check_launch();
+  }
+
+  // The GPU computes a scene's WLLSE when the edge radius is below the
+  // bucket width and every wall lies inside the bucket grid (no clamped
+  // cell), so the 27-bucket stencil holds every wall within the radius.
+  bool edge_on_gpu(const mps_params& p, const mps_scene& sc_, int s) const {
+    const mps::Cst<real>& c = csts[s];
+    if (!(p.edge_factor * p.l0 < 0.9 * double(hks[s].db))) return false;
+    for (int i = 0; i < sc_.n; i++) {
+      if (!mps::is_wall_type(sc_.typ[i])) continue;
+      for (int a = 0; a < DIM; a++) {
+        const real t = (real(sc_.pos[a * (size_t)sc_.n + i]) - c.minc[a]) * c.dbinv;
+        if (!(t >= real(0) && t < real(c.nb[a] - 2))) return false;
+      }
+    }
+    return true;
+  }
+
+  int surface_edge_gpu(const std::vector<double>& p64, const std::vector<int>& wsid,
+                       const std::vector<double>& R2s, const std::vector<uint8_t>& todo) {
+    double *d64 = nullptr, *dR2 = nullptr;
+    int* dsid = nullptr;
+    uint8_t* dtodo = nullptr;
+    int8_t* dtyp = nullptr;
+    auto release = [&]() {
+      (void)hipFree(d64);
+      (void)hipFree(dR2);
+      (void)hipFree(dsid);
+      (void)hipFree(dtodo);
+      (void)hipFree(dtyp);
+    };
+    hipError_t e = hipMalloc(&d64, std::max<size_t>(p64.size(), 1) * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&dR2, R2s.size() * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&dsid, std::max<size_t>(wsid.size(), 1) * sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&dtodo, todo.size());
+    if (e == hipSuccess) e = hipMalloc(&dtyp, std::max(nw, 1));
+    if (e == hipSuccess) e = hipMemcpy(d64, p64.data(), p64.size() * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dR2, R2s.data(), R2s.size() * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dsid, wsid.data(), wsid.size() * sizeof(int), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dtodo, todo.data(), todo.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dtyp, wall_typ.data(), nw, hipMemcpyHostToDevice);
+    if (e == hipSuccess && nw > 0) {
+      hipLaunchKernelGGL((mps::k_surface_edge<real, DIM>), dim3(nblk(nw)), dim3(mps::kBlock), 0, stream,
+                         (const double*)d64, nw, (const int*)dsid, (const double*)dR2, (const uint8_t*)dtodo,
+                         (const V*)wpos, (const V*)wnrm, (const int8_t*)dtyp, (const int*)wcs,
+                         (const mps::Cst<real>*)dtab, wedge, wcls);
+      e = hipGetLastError();
+      if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    }
+    release();
+    if (e != hipSuccess) return set_err(MPS_E_HIP, std::string("Surface_Edge on the GPU: ") + hipGetErrorString(e));
+    return MPS_OK;
+  }
+
+  int get_edge(int8_t* edge) override {
+    std::vector<int8_t> he;
+    int rc;
+    if ((rc = d2h(he, wedge, nw))) return rc;
+    HIP_TRY(hipStreamSynchronize(stream));
+    std::fill(edge, edge + n_total, int8_t(MPS_SURFACE));
+    for (int w = 0; w < nw; w++) edge[host_wall_ids[w]] = he[w];
+    return MPS_OK;
   }
 
   // exclusive scan of in[0..n) into out[0..n] (out[n] = total); in is
@@ -1450,6 +1531,11 @@ int mps_compute_constants(const mps_params* p, const mps_scene* s, mps_constants
   out->n_fluid = s->n - nw;
   out->n_wall = nw;
   return MPS_OK;
+}
+
+int mps_get_edge(mps_handle* h, int8_t* edge) {
+  if (!h || !edge) return set_err(MPS_E_ARG, "bad argument");
+  return h->impl->get_edge(edge);
 }
 
 int mps_compute_edge(const mps_params* p, const mps_scene* s, int8_t* edge) {

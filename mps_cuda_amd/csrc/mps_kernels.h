@@ -438,6 +438,52 @@ __global__ void k_copy_tail(const int* __restrict__ cs, int tot, int n,
 }
 
 // ---------------------------------------------------------------------------
+// d_Surface_Edge (kernel.cu:553-668) over the sorted wall set, once at set-up.
+// A wall is an Edge when a wall of its own type within the edge radius has a
+// different normal.  Distances are taken in FP64 from the scene's FP64
+// positions (p64, [3][nw] in sorted order), normals are compared in `real`,
+// as the host restatement does, so the flags are the same bit for bit; the
+// 27-bucket stencil holds every wall within the radius (the host only sends
+// scenes whose radius is below the bucket width and whose walls all lie
+// inside the bucket grid).  Sets the edge bit (0x80) of the class byte too.
+template <class real, int DIM>
+__global__ void k_surface_edge(const double* __restrict__ p64, int nw, const int* __restrict__ wsid,
+                               const double* __restrict__ R2s, const uint8_t* __restrict__ todo,
+                               const Vec<real, DIM>* __restrict__ wpos, const Vec<real, DIM>* __restrict__ wnrm,
+                               const int8_t* __restrict__ wtyp, const int* __restrict__ wcs,
+                               const Cst<real>* __restrict__ tab, int8_t* __restrict__ wedge,
+                               uint8_t* __restrict__ wcls) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nw) return;
+  const int s = wsid[i];
+  if (!todo[s]) return;
+  const Cst<real> c = tab[s];
+  const double R2 = R2s[s];
+  const double x = p64[i], y = p64[(size_t)nw + i], z = DIM == 3 ? p64[2 * (size_t)nw + i] : 0.0;
+  const Vec<real, DIM> n = wnrm[i];
+  const int t = wtyp[i];
+  const Cell<real, DIM> q = cell_of<real, DIM>(wpos[i], c);
+  bool edge = false;
+  const int zlo = DIM == 3 ? q.iz - 1 : 0, zhi = DIM == 3 ? q.iz + 1 : 0;
+  for (int jz = zlo; jz <= zhi; jz++)
+    for (int jy = q.iy - 1; jy <= q.iy + 1; jy++) {
+      const int b = wcs[cell_key(q.ix - 1, jy, jz, c)], e = wcs[cell_key(q.ix + 1, jy, jz, c) + 1];
+      for (int j = b; j < e; j++) {
+        if (j == i || wtyp[j] != t) continue;
+        const double px = x - p64[j], py = y - p64[(size_t)nw + j];
+        const double pz = DIM == 3 ? z - p64[2 * (size_t)nw + j] : 0.0;
+        const double d2 = px * px + py * py + pz * pz;
+        const Vec<real, DIM> nj = wnrm[j];
+        if (d2 < R2 && (nj.x != n.x || nj.y != n.y || vz(nj) != vz(n))) edge = true;
+      }
+    }
+  if (edge) {
+    wedge[i] = kEdge;
+    wcls[i] = uint8_t(wcls[i] | 0x80);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Mirror generation + cull: d_GenMRR_nonslip (kernel.cu:720-893) followed by
 // d_MRRinout (kernel.cu:932-985) for each generated mirror.
 

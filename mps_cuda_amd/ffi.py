@@ -94,7 +94,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
                "mps_compute_constants", "mps_compute_edge", "mps_probe_arith", "mps_sync",
                "mps_set_stream", "mps_timing_enable", "mps_timing_read", "mps_create_batch",
                "mps_get_scene_constants", "mps_set_graph", "mps_set_launch",
-               "mps_adjoint_forward", "mps_adjoint_backward", "mps_adjoint_release"):
+               "mps_adjoint_forward", "mps_adjoint_backward", "mps_adjoint_release",
+               "mps_get_edge"):
         getattr(lib, fn).restype = C.c_int
     lib.mps_adjoint_forward.argtypes = [C.c_void_p, C.c_int, C.c_int]
     lib.mps_adjoint_backward.argtypes = [C.c_void_p] * 6
@@ -104,6 +105,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.mps_set_graph.argtypes = [C.c_void_p, C.c_int]
     lib.mps_set_launch.argtypes = [C.c_void_p, C.c_int, C.c_int]
     lib.mps_sync.argtypes = [C.c_void_p]
+    lib.mps_get_edge.argtypes = [C.c_void_p, C.c_void_p]
     lib.mps_set_stream.argtypes = [C.c_void_p, C.c_void_p]
     lib.mps_timing_enable.argtypes = [C.c_void_p, C.c_int]
     lib.mps_timing_read.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
@@ -322,6 +324,12 @@ class Handle:
                       (pav, np.float64)):
             arrs.append(None if a is None else np.ascontiguousarray(a, dtype=dt))
         self._check(self._f("set_state")(self._h, *(_ptr(a) for a in arrs)))
+
+    def edge(self) -> np.ndarray:
+        """WLLSE in creation order, as computed when the handle was created."""
+        e = np.zeros(self.n, np.int8)
+        self._check(self.lib.mps_get_edge(self._h, e.ctypes.data_as(C.c_void_p)))
+        return e
 
     def buckets(self, which=BKT_FLUID):
         cnt = C.c_int()
