@@ -62,6 +62,15 @@ enum mps_pmax_mode {
   MPS_PMAX_REFERENCE = 2, /* Pmax = rho*G.y*H, the reference's literal value */
 };
 
+/* Pressure gradient model of d_PrsGrdTrm. */
+enum mps_grad_model {
+  MPS_GRAD_STANDARD = 0, /* dim/n0 * sum w (P_j - P_min) r_ij, the active kernel (kernel.cu:1651-1794) */
+  MPS_GRAD_TENSOR = 1,   /* moment-matrix corrected gradient of the compiled-out kernel
+                            (kernel.cu:1434-1649): M^-1 * sum w (P_j - P_min) r_ij / n0 with
+                            M = sum w r_ij r_ij^T / n0; 2-D uses the 2x2 block of M.  Not
+                            available to the adjoint. */
+};
+
 /* Physical and numerical parameters: the 17 values of MPS_physical.txt
  * (kernel.cu:12-22) plus the constants the reference hard-codes
  * (const.cuh:38-46, kernel.cu:309-315, 1184).  Fill with mps_params_default
@@ -70,7 +79,7 @@ typedef struct mps_params {
   int dim;            /* 2 or 3 (the reference is 3-D only, kernel.cu:341-343) */
   int precision;      /* enum mps_precision; `real` of const.cuh:52-53 */
   int pmax_mode;      /* enum mps_pmax_mode */
-  int reserved0;
+  int grad_model;     /* enum mps_grad_model */
   double l0;          /* PCL_DST, particle spacing */
   double range_min[3];/* MPS_physical.txt line 1 (before the re expansion, kernel.cu:26-31) */
   double range_max[3];/* MPS_physical.txt line 2 */

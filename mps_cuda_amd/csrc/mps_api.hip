@@ -199,6 +199,7 @@ struct Impl : ImplBase {
   int8_t* wedge = nullptr;
   uint8_t* wcls = nullptr;  // normal class (bits 0-5) | edge flag (bit 7)
   bool use_cls = false;     // at most 64 distinct wall normals
+  bool tensor = false;      // grad_model == MPS_GRAD_TENSOR (all scenes)
   int* wcs = nullptr;
   real *wprs = nullptr, *wpav = nullptr;
   // mirrors
@@ -277,6 +278,7 @@ struct Impl : ImplBase {
 
   int init(int nscene, const mps_params* ps, const mps_scene* ss) override {
     S = nscene;
+    tensor = ps[0].grad_model == MPS_GRAD_TENSOR;
     std::string err;
     int rc;
     hks.resize(S);
@@ -688,6 +690,22 @@ struct Impl : ImplBase {
   // d_PrsGrdTrm (+ d_UpPcl2 when fuse: new pos/vel/prs swapped in, pav += prs).
   void run_grad(bool fuse) {
     const dim3 SB(block), SG(sb_grid);
+    if (tensor) {
+      if (fuse) {
+        TLAUNCH("grad_up2", (mps::k_grad_list<real, DIM, true, true>), SG, SB, 0, stream, fpos[0], fvel[0],
+                fprs[0], (const int*)fcs, nf, nlist, ncount, mpos, mprs, mcs, (const uint8_t*)mnear, sc(),
+                (V*)nullptr, fpos[1], fvel[1], fprs[1], fpav[0], ftyp[0]);
+        copy_tail(fpos[0], fpos[1], fvel[0], fvel[1], fprs[0], fprs[1], fpav[0]);
+        std::swap(fpos[0], fpos[1]);
+        std::swap(fvel[0], fvel[1]);
+        std::swap(fprs[0], fprs[1]);
+      } else {
+        TLAUNCH("grad", (mps::k_grad_list<real, DIM, false, true>), SG, SB, 0, stream, fpos[0], fvel[0],
+                fprs[0], (const int*)fcs, nf, nlist, ncount, mpos, mprs, mcs, (const uint8_t*)mnear, sc(),
+                facc[0], (V*)nullptr, (V*)nullptr, (real*)nullptr, (real*)nullptr, (int8_t*)nullptr);
+      }
+      return;
+    }
     if (fuse) {
       TLAUNCH("grad_up2", (mps::k_grad_list<real, DIM, true>), SG, SB, 0, stream, fpos[0], fvel[0],
               fprs[0], (const int*)fcs, nf, nlist, ncount, mpos, mprs, mcs, (const uint8_t*)mnear, sc(),
@@ -1103,6 +1121,7 @@ struct Impl : ImplBase {
   // `every` steps (and the final one) for the reverse sweep.
   int adj_forward(int nsteps, int every) override {
     if (S != 1) return set_err(MPS_E_STATE, "the adjoint runs on single-scene handles");
+    if (tensor) return set_err(MPS_E_ARG, "the adjoint supports grad_model MPS_GRAD_STANDARD only");
     if (nsteps < 1 || every < 1) return set_err(MPS_E_ARG, "need nsteps >= 1 and checkpoint_every >= 1");
     if (nf == 0) return set_err(MPS_E_STATE, "no fluid particles");
     drop_graph();
@@ -1349,8 +1368,8 @@ int create_handle(int nscene, const mps_params* p, const mps_scene* s, ImplBase*
   for (int k = 0; k < nscene; k++) {
     int rc = mps::validate(p + k, s + k, err);
     if (rc != MPS_OK) return set_err(rc, nscene > 1 ? "scene " + std::to_string(k) + ": " + err : err);
-    if (p[k].dim != p[0].dim || p[k].precision != p[0].precision)
-      return set_err(MPS_E_ARG, "all scenes of a batch need the same dim and precision");
+    if (p[k].dim != p[0].dim || p[k].precision != p[0].precision || p[k].grad_model != p[0].grad_model)
+      return set_err(MPS_E_ARG, "all scenes of a batch need the same dim, precision and grad_model");
   }
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)

@@ -15,6 +15,8 @@ int validate(const mps_params* p, const mps_scene* s, std::string& err) {
   if (p->dim != 2 && p->dim != 3) return err = "dim must be 2 or 3", MPS_E_ARG;
   if (p->precision != MPS_FP32 && p->precision != MPS_FP64)
     return err = "precision must be MPS_FP32 or MPS_FP64", MPS_E_ARG;
+  if (p->grad_model != MPS_GRAD_STANDARD && p->grad_model != MPS_GRAD_TENSOR)
+    return err = "grad_model must be MPS_GRAD_STANDARD or MPS_GRAD_TENSOR", MPS_E_ARG;
   if (!(p->l0 > 0)) return err = "l0 must be positive", MPS_E_ARG;
   if (!(p->re_factor > 0) || !(p->rp_factor > 0) || p->rp_factor > p->re_factor)
     return err = "need 0 < rp_factor <= re_factor", MPS_E_ARG;

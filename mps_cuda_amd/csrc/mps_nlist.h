@@ -392,8 +392,16 @@ __global__ __launch_bounds__(kBlock) void k_prs_list(
 }
 
 // ---------------------------------------------------------------------------
-// d_PrsGrdTrm + d_UpPcl2 (kernel.cu:1651-1829) over the list.
-template <class real, int DIM, bool FUSE>
+// d_PrsGrdTrm + d_UpPcl2 (kernel.cu:1651-1829) over the list.  TENSOR
+// selects the compiled-out moment-matrix gradient (kernel.cu:1434-1649): the
+// nine sums M[a][b] += (w q_a) q_b kept apart (M[0][1] and M[1][0] round
+// differently), R[a] += (w (P_j - P_min)) q_a, both times 1/n0_grad, then
+// Acc = invro * M^-1 R by cofactors over one reciprocal determinant (2-D:
+// the 2x2 block).  When the scaled M is all zero the reference returns
+// without writing Acc, which still holds ChkCol's new velocity: the fused
+// update then uses that velocity as the acceleration, the phase kernel
+// leaves acc_out as ChkCol wrote it.
+template <class real, int DIM, bool FUSE, bool TENSOR = false>
 __global__ __launch_bounds__(kBlock) void k_grad_list(
     const Vec<real, DIM>* __restrict__ pos, const Vec<real, DIM>* __restrict__ vel,
     const real* __restrict__ prs, const int* __restrict__ fcs, int n_alloc,
@@ -427,6 +435,8 @@ __global__ __launch_bounds__(kBlock) void k_grad_list(
         if (norm2<real, DIM>(pj.x - px, pj.y - py, vz(pj) - pz) < rp2 && pr < Prs_min) Prs_min = pr;
       },
       prs);
+  // moment matrix (TENSOR): row-major, the 2x2 block in 2-D
+  real m00 = 0, m01 = 0, m02 = 0, m10 = 0, m11 = 0, m12 = 0, m20 = 0, m21 = 0, m22 = 0;
   auto add = [&](const Vec<real, DIM>& pj, real pr) {
     const real qx = pj.x - px, qy = pj.y - py, qz = vz(pj) - pz;
     const real dist2 = norm2<real, DIM>(qx, qy, qz);
@@ -434,9 +444,28 @@ __global__ __launch_bounds__(kBlock) void k_grad_list(
     const real dist = sqrt(dist2);
     const real w = ((rp / dist) - (dist / rp)) / dist2;
     const real d = pr - Prs_min;
-    ax += d * w * qx;
-    ay += d * w * qy;
-    if constexpr (DIM == 3) az += d * w * qz;
+    if constexpr (TENSOR) {
+      const real wx = w * qx, wy = w * qy, wd = w * d;
+      m00 += wx * qx;
+      m01 += wx * qy;
+      m10 += wy * qx;
+      m11 += wy * qy;
+      ax += wd * qx;
+      ay += wd * qy;
+      if constexpr (DIM == 3) {
+        const real wz = w * qz;
+        m02 += wx * qz;
+        m12 += wy * qz;
+        m20 += wz * qx;
+        m21 += wz * qy;
+        m22 += wz * qz;
+        az += wd * qz;
+      }
+    } else {
+      ax += d * w * qx;
+      ay += d * w * qy;
+      if constexpr (DIM == 3) az += d * w * qz;
+    }
   };
   if (act)
   for_near<real, DIM>(
@@ -445,9 +474,50 @@ __global__ __launch_bounds__(kBlock) void k_grad_list(
   for_mirrors_near<real, DIM>(stage, act && mnear[cell_key(q.ix, q.iy, q.iz, c)], px, py, pz, mpos,
                               mcs, q, c, [&](int j, const Vec<real, DIM>& pj) { add(pj, mprs[j]); });
   if (!act) return;
-  const real accx = c.invro * ax * c.grad_a;
-  const real accy = c.invro * ay * c.grad_a;
-  const real accz = DIM == 3 ? c.invro * az * c.grad_a : real(0);
+  real accx, accy, accz;
+  if constexpr (TENSOR) {
+    const real invn0 = real(1) / c.n0_grad;
+    m00 = invn0 * m00, m01 = invn0 * m01, m10 = invn0 * m10, m11 = invn0 * m11;
+    if constexpr (DIM == 3)
+      m02 = invn0 * m02, m12 = invn0 * m12, m20 = invn0 * m20, m21 = invn0 * m21, m22 = invn0 * m22;
+    bool zero = m00 == real(0) && m01 == real(0) && m10 == real(0) && m11 == real(0);
+    if constexpr (DIM == 3)
+      zero = zero && m02 == real(0) && m12 == real(0) && m20 == real(0) && m21 == real(0) && m22 == real(0);
+    if (zero) {
+      if constexpr (!FUSE) return;  // acc_out keeps ChkCol's velocity
+      const Vec<real, DIM> v = vel[i];
+      accx = v.x, accy = v.y, accz = vz(v);
+    } else {
+      const real r0 = invn0 * ax, r1 = invn0 * ay;
+      if constexpr (DIM == 3) {
+        const real r2 = invn0 * az;
+        const real det = real(1) / ((m00 * m11 * m22 + m01 * m12 * m20 + m02 * m10 * m21) -
+                                    (m02 * m11 * m20 + m00 * m12 * m21 + m01 * m10 * m22));
+        const real l0 = det * (m11 * m22 - m12 * m21);
+        const real l1 = det * -(m01 * m22 - m02 * m21);
+        const real l2 = det * (m01 * m12 - m02 * m11);
+        const real l3 = det * -(m10 * m22 - m12 * m20);
+        const real l4 = det * (m00 * m22 - m02 * m20);
+        const real l5 = det * -(m00 * m12 - m02 * m10);
+        const real l6 = det * (m10 * m21 - m11 * m20);
+        const real l7 = det * -(m00 * m21 - m01 * m20);
+        const real l8 = det * (m00 * m11 - m01 * m10);
+        accx = c.invro * (l0 * r0 + l1 * r1 + l2 * r2);
+        accy = c.invro * (l3 * r0 + l4 * r1 + l5 * r2);
+        accz = c.invro * (l6 * r0 + l7 * r1 + l8 * r2);
+      } else {
+        const real det = real(1) / (m00 * m11 - m01 * m10);
+        const real l0 = det * m11, l1 = det * -m01, l3 = det * -m10, l4 = det * m00;
+        accx = c.invro * (l0 * r0 + l1 * r1);
+        accy = c.invro * (l3 * r0 + l4 * r1);
+        accz = real(0);
+      }
+    }
+  } else {
+    accx = c.invro * ax * c.grad_a;
+    accy = c.invro * ay * c.grad_a;
+    accz = DIM == 3 ? c.invro * az * c.grad_a : real(0);
+  }
   if constexpr (!FUSE) {
     acc_out[i] = mkvec<real, DIM>(accx, accy, accz);
   } else {

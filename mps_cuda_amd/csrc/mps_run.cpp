@@ -7,7 +7,8 @@
 //   mps_run --dambreak cfg0|cfg1|cfg2a|small2d|small3d [--steps N] [--every K]
 //       runs a built-in dam break for N steps (output every K steps).
 //   --write-reference DIR  writes the scene as MPS_physical.txt + initial.txt
-//   --precision fp32|fp64, --pmax none|absg|reference, --c C, --dt DT,
+//   --precision fp32|fp64, --pmax none|absg|reference, --grad standard|tensor,
+//   --c C, --dt DT,
 //   --graph K (replay K-step hipGraphs), --no-output, --dry-run (host only:
 //   print the constants and exit).
 #include <algorithm>
@@ -39,7 +40,7 @@ struct Phys {
 
 struct Opts {
   std::string physical, initial, dambreak, out = "vtk_cuda", write_ref;
-  int precision = -1, pmax = -1, steps = -1, every = 0, graph = 0;
+  int precision = -1, pmax = -1, grad = MPS_GRAD_STANDARD, steps = -1, every = 0, graph = 0;
   double c = -1, dt = -1;
   bool output = true, dry = false;
 };
@@ -332,7 +333,8 @@ int usage() {
   std::fprintf(stderr,
                "usage: mps_run (--physical F --initial F | --dambreak cfg0|cfg1|cfg2a|small2d|small3d)\n"
                "               [--out DIR] [--steps N] [--every K] [--precision fp32|fp64]\n"
-               "               [--pmax none|absg|reference] [--c C] [--dt DT] [--graph K]\n"
+               "               [--pmax none|absg|reference] [--grad standard|tensor]\n"
+               "               [--c C] [--dt DT] [--graph K]\n"
                "               [--no-output] [--dry-run] [--write-reference DIR]\n");
   return 2;
 }
@@ -360,7 +362,11 @@ int main(int argc, char** argv) {
     else if (a == "--c") o.c = std::atof(v);
     else if (a == "--dt") o.dt = std::atof(v);
     else if (a == "--precision") o.precision = std::string(v) == "fp64" ? MPS_FP64 : MPS_FP32;
-    else if (a == "--pmax")
+    else if (a == "--grad") {
+      if (std::string(v) == "tensor") o.grad = MPS_GRAD_TENSOR;
+      else if (std::string(v) == "standard") o.grad = MPS_GRAD_STANDARD;
+      else return usage();
+    } else if (a == "--pmax")
       o.pmax = std::string(v) == "none" ? MPS_PMAX_NONE : std::string(v) == "absg" ? MPS_PMAX_ABS_G
                                                                                   : MPS_PMAX_REFERENCE;
     else return usage();
@@ -418,6 +424,7 @@ int main(int argc, char** argv) {
   p.dt = o.dt > 0 ? o.dt : dt;
   if (o.c > 0) p.c = o.c;
   if (o.pmax >= 0) p.pmax_mode = o.pmax;
+  p.grad_model = o.grad;
 
   if (!o.write_ref.empty()) {
     if (!write_reference(o.write_ref, ph, s)) return fail("cannot write the reference-format files");

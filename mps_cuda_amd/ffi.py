@@ -21,6 +21,7 @@ SURFACE, EDGE = 0, 1
 NUM_MRR = 10
 FP32, FP64 = 0, 1
 PMAX_NONE, PMAX_ABS_G, PMAX_REFERENCE = 0, 1, 2
+GRAD_STANDARD, GRAD_TENSOR = 0, 1
 PH_MKBKT, PH_MIRROR, PH_VSC, PH_UP1, PH_COL, PH_PRS, PH_GRAD, PH_UP2 = range(8)
 BKT_FLUID, BKT_WALL, BKT_MIRROR = 0, 1, 2
 STEP_PHASES = [PH_MKBKT, PH_MIRROR, PH_VSC, PH_UP1, PH_MKBKT, PH_COL, PH_MKBKT, PH_PRS,
@@ -31,7 +32,7 @@ _d3 = C.c_double * 3
 
 class Params(C.Structure):
     _fields_ = [
-        ("dim", C.c_int), ("precision", C.c_int), ("pmax_mode", C.c_int), ("reserved0", C.c_int),
+        ("dim", C.c_int), ("precision", C.c_int), ("pmax_mode", C.c_int), ("grad_model", C.c_int),
         ("l0", C.c_double), ("range_min", _d3), ("range_max", _d3), ("gravity", _d3),
         ("rho", C.c_double), ("nu", C.c_double), ("c", C.c_double), ("mach", C.c_double),
         ("dt", C.c_double), ("restitution", C.c_double), ("dst_lmt_rat", C.c_double),

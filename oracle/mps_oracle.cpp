@@ -678,6 +678,7 @@ struct Sim : Base {
 
   // ---- d_PrsGrdTrm, active version (kernel.cu:1651-1794) --------------------
   void prs_grd() {
+    if (prm.grad_model == MPS_GRAD_TENSOR) return prs_grd_tensor();
 #pragma omp parallel for schedule(dynamic, 256)
     for (int i = 0; i < nP; i++) {
       if (Typ[i] != FLD) continue;
@@ -728,6 +729,87 @@ struct Sim : Base {
       Acc[0][i] = invro * ax * A;
       Acc[1][i] = invro * ay * A;
       if (dim == 3) Acc[2][i] = invro * az * A;
+    }
+  }
+
+  // ---- d_PrsGrdTrm, compiled-out tensor version (kernel.cu:1434-1649) -------
+  // P_min as in the active version (kernel.cu:1480-1511).  Over the same
+  // fluid and mirror neighbours (d2 < rp2) it sums the moment matrix
+  // M[a][b] += (w q_a) q_b, all nine entries separately (M[0][1] and M[1][0]
+  // round differently), and R[a] += (w (P_j - P_min)) q_a
+  // (kernel.cu:1538-1549, 1579-1590); both are scaled by 1/n0_grad
+  // (kernel.cu:1601-1609, 1625-1627).  A particle whose scaled M is all zero
+  // returns without writing Acc (kernel.cu:1611), which then still holds the
+  // velocity ChkCol left there.  Otherwise M is inverted by cofactors over
+  // one reciprocal determinant (kernel.cu:1613-1622) and
+  // Acc = invro * (M^-1 R) (kernel.cu:1631-1633).
+  // 2-D: the reference is 3-D only; here M is the 2x2 block (xx, xy, yx, yy),
+  // the zero test covers those four entries, and the inverse is the same
+  // cofactor formula, which equals the 3x3 formula with the z row and column
+  // of M set to the identity.
+  void prs_grd_tensor() {
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int i = 0; i < nP; i++) {
+      if (Typ[i] != FLD) continue;
+      const real invn0 = real(1) / n0_grad;
+      const real invro = real(-1) / Dns;
+      const real px = Pos[0][i], py = Pos[1][i], pz = dim == 3 ? Pos[2][i] : real(0);
+      real Prs_min = Prs[i];
+      int ix, iy, iz;
+      cell3(px, py, pz, ix, iy, iz);
+      stencil(bfst, nxt, ix, iy, iz, [&](int j) {
+        if (j == i || Typ[j] != FLD) return;
+        real qx = Pos[0][j] - px, qy = Pos[1][j] - py;
+        real qz = dim == 3 ? Pos[2][j] - pz : real(0);
+        real dist2 = dim == 3 ? qx * qx + qy * qy + qz * qz : qx * qx + qy * qy;
+        if (dist2 < rp2 && Prs[j] < Prs_min) Prs_min = Prs[j];
+      });
+      real m[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, rr[3] = {0, 0, 0};
+      auto add = [&](real qx, real qy, real qz, real pj) {
+        real dist2 = dim == 3 ? qx * qx + qy * qy + qz * qz : qx * qx + qy * qy;
+        if (!(dist2 < rp2)) return;
+        const real dist = std::sqrt(dist2);
+        const real w = WEI_grad(dist, rp) / dist2;
+        const real d = pj - Prs_min;
+        const real q[3] = {qx, qy, qz};
+        for (int a = 0; a < dim; a++)
+          for (int b = 0; b < dim; b++) m[3 * a + b] += w * q[a] * q[b];
+        for (int a = 0; a < dim; a++) rr[a] += w * d * q[a];
+      };
+      stencil(bfst, nxt, ix, iy, iz, [&](int j) {
+        if (j == i || Typ[j] != FLD) return;
+        add(Pos[0][j] - px, Pos[1][j] - py, dim == 3 ? Pos[2][j] - pz : real(0), Prs[j]);
+      });
+      stencil(bfstM, nxtM, ix, iy, iz, [&](int j) {
+        add(PosM[0][j] - px, PosM[1][j] - py, dim == 3 ? PosM[2][j] - pz : real(0), PrsM[j]);
+      });
+      for (int k = 0; k < 9; k++) m[k] = invn0 * m[k];
+      bool zero = true;
+      for (int a = 0; a < dim; a++)
+        for (int b = 0; b < dim; b++) zero = zero && m[3 * a + b] == real(0);
+      if (zero) continue;  // Acc keeps ChkCol's velocity
+      for (int a = 0; a < dim; a++) rr[a] = invn0 * rr[a];
+      if (dim == 3) {
+        const real det = real(1) / ((m[0] * m[4] * m[8] + m[1] * m[5] * m[6] + m[2] * m[3] * m[7]) -
+                                    (m[2] * m[4] * m[6] + m[0] * m[5] * m[7] + m[1] * m[3] * m[8]));
+        const real l0 = det * (m[4] * m[8] - m[5] * m[7]);
+        const real l1 = det * -(m[1] * m[8] - m[2] * m[7]);
+        const real l2 = det * (m[1] * m[5] - m[2] * m[4]);
+        const real l3 = det * -(m[3] * m[8] - m[5] * m[6]);
+        const real l4 = det * (m[0] * m[8] - m[2] * m[6]);
+        const real l5 = det * -(m[0] * m[5] - m[2] * m[3]);
+        const real l6 = det * (m[3] * m[7] - m[4] * m[6]);
+        const real l7 = det * -(m[0] * m[7] - m[1] * m[6]);
+        const real l8 = det * (m[0] * m[4] - m[1] * m[3]);
+        Acc[0][i] = invro * (l0 * rr[0] + l1 * rr[1] + l2 * rr[2]);
+        Acc[1][i] = invro * (l3 * rr[0] + l4 * rr[1] + l5 * rr[2]);
+        Acc[2][i] = invro * (l6 * rr[0] + l7 * rr[1] + l8 * rr[2]);
+      } else {
+        const real det = real(1) / (m[0] * m[4] - m[1] * m[3]);
+        const real l0 = det * m[4], l1 = det * -m[1], l3 = det * -m[3], l4 = det * m[0];
+        Acc[0][i] = invro * (l0 * rr[0] + l1 * rr[1]);
+        Acc[1][i] = invro * (l3 * rr[0] + l4 * rr[1]);
+      }
     }
   }
 
@@ -905,6 +987,8 @@ extern "C" {
 int orc_create(const mps_params* p, const mps_scene* s, orc_handle** out) {
   if (!p || !s || !out) return fail(MPS_E_ARG, "null argument");
   if (p->dim != 2 && p->dim != 3) return fail(MPS_E_ARG, "dim must be 2 or 3");
+  if (p->grad_model != MPS_GRAD_STANDARD && p->grad_model != MPS_GRAD_TENSOR)
+    return fail(MPS_E_ARG, "bad grad_model");
   if (s->n <= 0 || !s->typ || !s->pos) return fail(MPS_E_ARG, "empty scene");
   for (int i = 0; i < s->n; i++) {
     int t = s->typ[i];

@@ -116,7 +116,7 @@ def test_device_entry_points_fail_loudly_without_a_gpu():
 def test_batch_checks_every_scene_before_touching_the_device():
     s2, p2 = configs.small2d("fp32")
     s3, p3 = configs.small3d("fp32")
-    with pytest.raises(ffi.MPSError, match="same dim and precision"):
+    with pytest.raises(ffi.MPSError, match="same dim, precision"):
         ffi.Handle([p2, p3], [s2, s3])
     bad = ffi.SceneArrays(typ=s2.typ.copy(), pos=s2.pos, normal=s2.normal,
                           range_min=s2.range_min, range_max=s2.range_max)

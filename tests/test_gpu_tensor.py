@@ -1,0 +1,125 @@
+"""Tensor-corrected pressure gradient (grad_model = MPS_GRAD_TENSOR, the
+compiled-out d_PrsGrdTrm of kernel.cu:1434-1649) on the GPU against the CPU
+oracle: bit-exact phase by phase and over fused steps, in FP64 and FP32, 2-D
+and 3-D; exact for linear pressure on irregular neighbourhoods; the
+no-neighbour return that keeps ChkCol's velocity; batches; the adjoint
+refuses the model."""
+import numpy as np
+import pytest
+
+from mps_cuda_amd import configs, ffi
+from tests.parity import assert_buckets_equal, assert_mirrors_equal, assert_states_equal
+from tests.test_oracle import L0, params_of, scene_of, tensor_clusters
+
+pytestmark = pytest.mark.gpu
+
+T = dict(grad_model=ffi.GRAD_TENSOR)
+
+
+def _pair(oracle_mod, scene, params):
+    print(f"[case] tensor n={scene.n} dim={params.dim} prec={params.precision}", flush=True)
+    return ffi.Handle(params, scene), oracle_mod.Oracle(params, scene)
+
+
+@pytest.mark.parametrize("name,make", [
+    ("small2d-fp64", lambda: configs.small2d("fp64", **T)),
+    ("small2d-fp32", lambda: configs.small2d("fp32", **T)),
+    ("small3d-fp64", lambda: configs.small3d("fp64", **T)),
+    ("small3d-fp32", lambda: configs.small3d("fp32", **T)),
+])
+def test_tensor_phase_by_phase_bit_exact(oracle_mod, name, make):
+    scene, params = make()
+    g, o = _pair(oracle_mod, scene, params)
+    for step in range(3):
+        for k, ph in enumerate(ffi.STEP_PHASES):
+            g.phase(ph)
+            o.phase(ph)
+            where = f"{name} step {step} phase#{k} ({ph})"
+            assert_states_equal(g.state(), o.state(), where)
+            if ph == ffi.PH_MIRROR:
+                assert_mirrors_equal(g, o, where)
+
+
+@pytest.mark.parametrize("name,make,steps", [
+    ("cfg0-fp64", lambda: configs.cfg0("fp64", **T), 200),
+    ("cfg0-fp32", lambda: configs.cfg0("fp32", **T), 200),
+    ("small3d-fp32", lambda: configs.small3d("fp32", **T), 100),
+    ("small3d-fp64", lambda: configs.small3d("fp64", **T), 100),
+])
+def test_tensor_fused_steps_bit_exact(oracle_mod, name, make, steps):
+    scene, params = make()
+    g, o = _pair(oracle_mod, scene, params)
+    done = 0
+    for chunk in (1, 9, steps - 10):
+        g.step(chunk)
+        o.step(chunk)
+        done += chunk
+        where = f"{name} after {done} steps"
+        assert_states_equal(g.state(), o.state(), where, fields=("typ", "pos", "vel", "prs", "pav", "nd"))
+        assert_buckets_equal(g, o, ffi.BKT_FLUID, where)
+
+
+@pytest.mark.parametrize("dim,prec", [(2, "fp64"), (3, "fp64"), (3, "fp32")])
+def test_tensor_linear_pressure_clusters(oracle_mod, dim, prec):
+    """Irregular half-space neighbourhoods (see tensor_clusters): the GPU's
+    centre accelerations equal the oracle's bit for bit and -g/rho."""
+    pts, prs, centres, grads = tensor_clusters(dim, 60, 3)
+    p = params_of(dim, prec, **T)
+    g, o = _pair(oracle_mod, scene_of(dim, pts), p)
+    for h in (g, o):
+        h.phase(ffi.PH_MKBKT)
+        h.set_state(prs=prs)
+        h.phase(ffi.PH_GRAD)
+    assert_states_equal(g.state(), o.state(), f"clusters dim {dim} {prec}", fields=("acc",))
+    acc = g.state()["acc"][:, centres].T
+    want = -grads / 1000.0
+    err = np.linalg.norm(acc - want, axis=1) / np.linalg.norm(want, axis=1)
+    assert err.max() < (1e-10 if prec == "fp64" else 2e-3), err.max()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_tensor_no_neighbour_return_in_fused_steps(oracle_mod, prec):
+    """A lattice block and lone moving particles: the lone ones take the
+    reference's return without writing Acc (velocity used as acceleration)."""
+    block = np.array([[x, y, z] for x in range(-3, 4) for y in range(-3, 4) for z in range(-3, 4)],
+                     float) * L0
+    lone = np.array([[0.3, 0.2, 0.1], [-0.4, 0.5, -0.3], [0.5, -0.4, 0.4]])
+    vel = np.zeros((len(block) + len(lone), 3))
+    vel[len(block):] = [[0.2, -0.1, 0.05], [-0.3, 0.0, 0.1], [0.0, 0.25, -0.2]]
+    scene = scene_of(3, np.concatenate([block, lone]), vel=vel)
+    g, o = _pair(oracle_mod, scene, params_of(3, prec, **T))
+    for chunk in (1, 4, 15):
+        g.step(chunk)
+        o.step(chunk)
+        assert_states_equal(g.state(), o.state(), f"lattice + lone {prec}",
+                            fields=("typ", "pos", "vel", "prs", "pav", "nd"))
+    st = g.state()
+    assert np.all(st["typ"] == ffi.FLD)
+
+
+def test_tensor_batch_equals_single_scenes():
+    scenes, params = configs.batch_small(4, "fp64")
+    for p in params:
+        p.grad_model = ffi.GRAD_TENSOR
+    b = ffi.Handle(params, scenes)
+    singles = [ffi.Handle(p, sc) for p, sc in zip(params, scenes)]
+    b.step(20)
+    st = b.state()
+    for k, h in enumerate(singles):
+        h.step(20)
+        a, e = b.offsets[k], b.offsets[k + 1]
+        assert_states_equal({f: v[..., a:e] for f, v in st.items()}, h.state(), f"batch scene {k}")
+
+
+def test_tensor_batch_needs_one_model():
+    scenes, params = configs.batch_small(2, "fp64")
+    params[1].grad_model = ffi.GRAD_TENSOR
+    with pytest.raises(ffi.MPSError, match="grad_model"):
+        ffi.Handle(params, scenes)
+
+
+def test_adjoint_refuses_the_tensor_model():
+    scene, params = configs.small2d("fp64", **T)
+    h = ffi.Handle(params, scene)
+    with pytest.raises(ffi.MPSError, match="grad_model"):
+        h.adjoint_forward(4, 2)

@@ -282,6 +282,94 @@ def test_gradient_uses_local_minimum_pressure(oracle_mod):
     assert acc[0, 1] == 0.0  # P_j - P_min = 0 for the high-pressure particle's neighbour
 
 
+# --- PrsGrdTrm, tensor model (compiled out in the reference, kernel.cu:1434-1649) ----
+
+def tensor_clusters(dim, n_clusters, seed):
+    """Clusters 12 l0 apart: a centre particle and 6-12 neighbours at 0.6-2.0 l0
+    (inside rp = 2.1 l0), all in the half-space g.u > 0.3 of the cluster's own
+    gradient g.  With P = P_c + g.(x - x_c) the centre holds its neighbourhood's
+    minimum pressure, so R = M g exactly and the corrected gradient returns g
+    whatever the (irregular) neighbourhood; the standard model does not."""
+    rng = np.random.default_rng(seed)
+    side = int(math.ceil(n_clusters ** (1 / dim)))
+    pts, prs, centres, grads = [], [], [], []
+    for k in range(n_clusters):
+        idx = np.unravel_index(k, (side,) * dim)
+        c = np.zeros(3)
+        c[:dim] = (np.array(idx) - side / 2) * 12 * L0
+        g = np.zeros(3)
+        g[:dim] = rng.normal(size=dim)
+        g *= rng.uniform(500, 5000) / np.linalg.norm(g)
+        gu = g / np.linalg.norm(g)
+        centres.append(len(pts))
+        grads.append(g)
+        pts.append(c)
+        prs.append(2000.0)
+        m = rng.integers(6, 13)
+        while m:
+            u = np.zeros(3)
+            u[:dim] = rng.normal(size=dim)
+            u /= np.linalg.norm(u)
+            if u @ gu <= 0.3:
+                continue
+            q = u * rng.uniform(0.6, 2.0) * L0
+            pts.append(c + q)
+            prs.append(2000.0 + g @ q)
+            m -= 1
+    return np.array(pts), np.array(prs), np.array(centres), np.array(grads)
+
+
+def gradient_acc(oracle_mod, dim, model, precision="fp64", seed=3):
+    pts, prs, centres, grads = tensor_clusters(dim, 60, seed)
+    o = oracle_mod.Oracle(params_of(dim, precision, grad_model=model), scene_of(dim, pts))
+    o.phase(ffi.PH_MKBKT)
+    o.set_state(prs=prs)
+    o.phase(ffi.PH_GRAD)
+    return o.state()["acc"][:, centres].T, grads
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_tensor_gradient_is_exact_for_linear_pressure(oracle_mod, dim):
+    acc, g = gradient_acc(oracle_mod, dim, ffi.GRAD_TENSOR)
+    want = -g / 1000.0  # invro * g with rho = 1000
+    err = np.linalg.norm(acc - want, axis=1) / np.linalg.norm(want, axis=1)
+    assert err.max() < 1e-10, err.max()
+    std, _ = gradient_acc(oracle_mod, dim, ffi.GRAD_STANDARD)
+    err_std = np.linalg.norm(std - want, axis=1) / np.linalg.norm(want, axis=1)
+    assert np.median(err_std) > 0.1  # the uncorrected gradient is far off here
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_tensor_gradient_fp32_tracks_fp64(oracle_mod, dim):
+    a64, g = gradient_acc(oracle_mod, dim, ffi.GRAD_TENSOR)
+    a32, _ = gradient_acc(oracle_mod, dim, ffi.GRAD_TENSOR, "fp32")
+    assert rel_inf(a32.T, a64.T) < 2e-3
+
+
+def test_tensor_gradient_without_neighbours_keeps_chkcol_velocity(oracle_mod):
+    """kernel.cu:1611 returns before writing Acc when M is all zero; Acc then
+    still holds the velocity ChkCol wrote (kernel.cu:1168-1300), so UpPcl2
+    adds v dt to the velocity.  The standard model writes an exact zero."""
+    v = [[0.3, -0.2, 0.1], [0.0, 0.0, 0.0]]
+    for model, want in ((ffi.GRAD_TENSOR, np.array(v[0])), (ffi.GRAD_STANDARD, np.zeros(3))):
+        # the second particle sets the fluid height (umax = sqrt(2 g H) = 3.1 m/s)
+        o = oracle_mod.Oracle(params_of(3, grad_model=model),
+                              scene_of(3, [[0, 0, 0], [0.2, 0.5, 0]], vel=v))
+        o.phase(ffi.PH_MKBKT)
+        o.phase(ffi.PH_COL)
+        o.phase(ffi.PH_GRAD)
+        acc = o.state()["acc"]
+        assert np.array_equal(acc[:, 0], want)
+        o.phase(ffi.PH_UP2)
+        vel = o.state()["vel"][:, 0]
+        assert np.array_equal(vel, np.array(v[0]) + want * 1e-4)
+
+
+def test_tensor_model_rejected_values(oracle_mod):
+    with pytest.raises(ffi.MPSError):
+        oracle_mod.Oracle(params_of(2, grad_model=7), scene_of(2, [[0, 0, 0]]))
+
+
 # --- mirrors (GenMRR_nonslip, MRRinout; kernel.cu:720-985) ----------------------------
 
 def test_plane_mirror_is_reflection_across_first_wall(oracle_mod):
