@@ -19,8 +19,11 @@ before the timed region).  Prints one JSON line:
                 stepped one after another with plain launches ("speedup")
   adjoint       the 300k scene for 200 steps with checkpoints every 25:
                 (forward with checkpoints + reverse sweep) / plain forward
+  tensor_gradient  the 300k scene with the tensor-corrected gradient: step rate
+                and the gradient kernel's average duration
 
 Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-batch]
+                       [--no-adjoint] [--no-tensor]
 """
 from __future__ import annotations
 
@@ -212,6 +215,37 @@ def adjoint_leg(steps: int, every: int, log) -> dict:
             "grad": {"nu": g["nu"], "c": g["c"], "vel0_norm": float(np.linalg.norm(g["vel0"]))}}
 
 
+def tensor_leg(steps: int, warmup: int, log) -> dict:
+    """cfg1 with the tensor-corrected gradient (grad_model MPS_GRAD_TENSOR):
+    step rate and the gradient kernel's average duration (HIP events)."""
+    from mps_cuda_amd import configs, ffi
+
+    scene, params = configs.cfg1("fp32", grad_model=ffi.GRAD_TENSOR)
+    h = ffi.Handle(params, scene)
+    h.step(warmup)
+    h.sync()
+    live0 = int((h.state()["typ"] == ffi.FLD).sum())
+    t0 = time.perf_counter()
+    h.step(steps)
+    h.sync()
+    elapsed = time.perf_counter() - t0
+    live = 0.5 * (live0 + int((h.state()["typ"] == ffi.FLD).sum()))
+    h.close()
+    # event-timed replay of the same steps, as in the plain leg
+    h = ffi.Handle(params, scene)
+    h.timing(True)
+    h.step(warmup)
+    h.timing(True)  # resets the accumulators
+    h.step(steps)
+    kt = h.timing_read()
+    h.close()
+    g = kt["grad_up2"]
+    log(f"[bench] tensor gradient: {steps} steps in {elapsed:.3f} s, grad_up2 {1e3 * g[0] / g[1]:.1f} us")
+    return {"workload": "cfg1: 3-D dam break, 300k fluid, FP32, tensor-corrected gradient",
+            "value": round(live * steps / elapsed, 1), "unit": "particle-steps/s",
+            "ms_per_step": round(1e3 * elapsed / steps, 4), "grad_up2_ms": round(g[0] / g[1], 5)}
+
+
 def plain_leg(args) -> dict:
     """cfg1: timed steps (event-free), then a deterministic replay with
     per-kernel HIP events for the roofline."""
@@ -277,6 +311,7 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=200)
     ap.add_argument("--seq-steps", type=int, default=10)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-tensor", action="store_true")
     ap.add_argument("--adjoint-steps", type=int, default=200)
     ap.add_argument("--adjoint-every", type=int, default=25)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -313,6 +348,10 @@ def main():
         adj = adjoint_leg(args.adjoint_steps, args.adjoint_every,
                           lambda msg: print(msg, file=sys.stderr, flush=True))
 
+    tensor = None
+    if not args.no_tensor:
+        tensor = tensor_leg(args.steps, args.warmup, lambda msg: print(msg, file=sys.stderr, flush=True))
+
     P = plain or dict(value=None, elapsed=None, nf0=None, nw=None, live=None, m=None, roofline=None, kt={})
     out = {
         "metric": METRIC, "value": None if plain is None else round(P["value"], 1), "unit": "particle-steps/s",
@@ -327,6 +366,7 @@ def main():
         "cpu_baseline": cpu,
         "batched": batch,
         "adjoint": adj,
+        "tensor_gradient": tensor,
         "kernels_ms": {k: round(v[0] / v[1], 5) for k, v in sorted(P["kt"].items(), key=lambda kv: -kv[1][0])},
     }
     print(json.dumps(out), flush=True)

@@ -11,8 +11,8 @@
  * lines it follows.  It is templated on `real` (double = FP64 oracle, float =
  * FP32 instantiation) and runs the per-particle loops with OpenMP.
  *
- * Decisions where the reference is undefined or hard-coded (numbered as the
- * reference quirks H1..H18 in DESIGN.md):
+ * Decisions where the reference is undefined or hard-coded (the H numbers
+ * label the reference quirks):
  *  - In-cell order (H4).  The reference appends with atomicExch, so the order
  *    inside a cell is not defined.  Here a cell's list holds its fluid
  *    particles in ascending index, then its wall particles in ascending index
@@ -28,6 +28,9 @@
  *    for every particle inside the domain; the reference would index out of
  *    bounds otherwise).
  *  - d_MRRinout is skipped for empty (GST) slots: it cannot change them.
+ *  - grad_model selects the compiled-out tensor gradient (kernel.cu:1434-1649)
+ *    instead of the active one; in 2-D it uses the 2x2 block of the moment
+ *    matrix (H19).
  *
  * Arithmetic: plain IEEE operations in the reference's order, built with
  * -ffp-contract=off; the only library call is sqrt (correctly rounded).
