@@ -367,7 +367,8 @@ def main():
         "batched": batch,
         "adjoint": adj,
         "tensor_gradient": tensor,
-        "kernels_ms": {k: round(v[0] / v[1], 5) for k, v in sorted(P["kt"].items(), key=lambda kv: -kv[1][0])},
+        "kernels_ms": {k: round(v[0] / v[1], 5) for k, v in sorted(P["kt"].items(), key=lambda kv: -kv[1][0])
+                       if v[1] > 0},
     }
     print(json.dumps(out), flush=True)
 

@@ -200,6 +200,11 @@ struct Impl : ImplBase {
   uint8_t* wcls = nullptr;  // normal class (bits 0-5) | edge flag (bit 7)
   bool use_cls = false;     // at most 64 distinct wall normals
   bool tensor = false;      // grad_model == MPS_GRAD_TENSOR (all scenes)
+  // Single-scene handles: the fused kernels that move particles also write
+  // the next bucket build's keys and counts (key, slot, cnt).  keys_ready:
+  // they are current for the next bucket(); cnt_dirty: cnt holds counts
+  // that no scan has consumed yet (cleared before any other count).
+  bool keys_ready = false, cnt_dirty = false;
   int* wcs = nullptr;
   real *wprs = nullptr, *wpav = nullptr;
   // mirrors
@@ -212,7 +217,8 @@ struct Impl : ImplBase {
   uint8_t *wnear = nullptr, *mnear = nullptr;       // any wall / mirror in a bucket's stencil
   // bucket scratch (shared by the fluid and mirror builds)
   int *key = nullptr, *slot = nullptr, *tmp = nullptr, *tmp_id = nullptr, *perm = nullptr, *cnt = nullptr;
-  int* part = nullptr;  // tile totals of the scan
+  mps::u64* part = nullptr;  // tile totals of the scan (int or packed pairs)
+  int* nstart = nullptr;     // [tot+2] starts of each bucket's near-wall particles
   // captured step graph
   int graph_steps = 0;
   hipGraph_t graph = nullptr;
@@ -365,7 +371,9 @@ struct Impl : ImplBase {
         (rc = alloc(&tmp_id, scr)) ||
         (rc = alloc(&perm, scr)) || (rc = alloc(&cnt, (size_t)tot + 1)))
       return rc;
-    if ((rc = alloc(&part, ((size_t)tot + 2 + mps::kScanTile - 1) / mps::kScanTile))) return rc;
+    if ((rc = alloc(&part, ((size_t)tot + 2 + mps::kScanTile - 1) / mps::kScanTile)) ||
+        (rc = alloc(&nstart, (size_t)tot + 2)))
+      return rc;
     HIP_TRY(hipMemcpy(dtab, csts.data(), S * sizeof(mps::Cst<real>), hipMemcpyHostToDevice));
 
     auto src = [&](int g, const double* mps_scene::*field, int k) {
@@ -530,14 +538,27 @@ struct Impl : ImplBase {
   }
 
   // exclusive scan of in[0..n) into out[0..n] (out[n] = total); in is
-  // zeroed as it is read (in = cnt, clear for the next count)
-  void scan(int* in, int n, int* out) {
+  // zeroed as it is read (in = cnt, clear for the next count).  With flag:
+  // also the exclusive scan of the counts of flagged indices (< nflag) into
+  // out2[0..n], its total into *total2.
+  void scan(int* in, int n, int* out, const uint8_t* flag = nullptr, int nflag = 0, int* out2 = nullptr,
+            int* total2 = nullptr) {
     const int nb = (n + 1 + mps::kScanTile - 1) / mps::kScanTile;
-    if (nb > 1)
-      hipLaunchKernelGGL(mps::k_scan_reduce, dim3(nb), dim3(mps::kBlock), 0, stream, (const int*)in, n,
-                         part);
-    hipLaunchKernelGGL(mps::k_scan_down, dim3(nb), dim3(mps::kBlock), 0, stream, in, n,
-                       nb > 1 ? (const int*)part : nullptr, out);
+    const dim3 G(nb), B(mps::kBlock);
+    if (flag) {
+      if (nb > 1)
+        hipLaunchKernelGGL(mps::k_scan_reduce<true>, G, B, 0, stream, (const int*)in, n, part, flag, nflag);
+      hipLaunchKernelGGL(mps::k_scan_down<true>, G, B, 0, stream, in, n,
+                         nb > 1 ? (const mps::u64*)part : nullptr, out, flag, nflag, out2, total2);
+    } else {
+      int* ipart = reinterpret_cast<int*>(part);
+      if (nb > 1)
+        hipLaunchKernelGGL(mps::k_scan_reduce<false>, G, B, 0, stream, (const int*)in, n, ipart,
+                           (const uint8_t*)nullptr, 0);
+      hipLaunchKernelGGL(mps::k_scan_down<false>, G, B, 0, stream, in, n,
+                         nb > 1 ? (const int*)ipart : nullptr, out, (const uint8_t*)nullptr, 0, (int*)nullptr,
+                         (int*)nullptr);
+    }
   }
 
   mps::FluidArrays<real, DIM> fa(int b) {
@@ -565,18 +586,28 @@ struct Impl : ImplBase {
       hipLaunchKernelGGL(mps::k_fill_int, dim3(1), B, 0, stream, blk_off, S + 1, 0);
       return MPS_OK;
     }
-    TLAUNCH("bucket_key", (mps::k_bkt_key<real, DIM>), dim3(nblk(nf)), B, 0, stream, fpos[0],
-            (const int8_t*)ftyp[0], (const int*)fsid[0], (const mps::Cst<real>*)dtab, nf, tot, key,
-            slot, cnt);
+    if (!keys_ready) {
+      if (cnt_dirty)
+        hipLaunchKernelGGL(mps::k_fill_int, dim3(nblk(tot + 1, 4096)), B, 0, stream, cnt, tot + 1, 0);
+      TLAUNCH("bucket_key", (mps::k_bkt_key<real, DIM>), dim3(nblk(nf)), B, 0, stream, fpos[0],
+              (const int8_t*)ftyp[0], (const int*)fsid[0], (const mps::Cst<real>*)dtab, nf, tot, key,
+              slot, cnt);
+    }
+    keys_ready = cnt_dirty = false;
+    // with wall normal classes the scan also gives the near-wall particles'
+    // list offsets (nstart) and their count (mcount) for the gather
     tick("bucket_scan");
-    scan(cnt, tot + 1, fcs);
+    if (use_cls) scan(cnt, tot + 1, fcs, wnear, tot, nstart, mcount);
+    else scan(cnt, tot + 1, fcs);
     tock();
     TLAUNCH("bucket_scatter", mps::k_bkt_scatter, dim3(nblk(nf)), B, 0, stream, key, slot, fid[0], fcs,
-            nf, (const int*)nullptr, 1, tmp, tmp_id);
-    TLAUNCH("bucket_rank", mps::k_bkt_rank, dim3(nblk(nf)), B, 0, stream, tmp, tmp_id, key, fcs, nf,
-            (const int*)nullptr, tot, perm, (int*)nullptr);
-    TLAUNCH("bucket_gather", (mps::k_gather_fluid<real, DIM>), dim3(nblk(nf)), B, 0, stream, perm, nf,
-            fa(0), fa(1));
+            nf, (const int*)nullptr, 1, tmp, tmp_id, (int*)nullptr);
+    TLAUNCH("bucket_rank", mps::k_bkt_rank, dim3(nblk(nf)), B, 0, stream, (const int*)tmp,
+            (const int*)tmp_id, (const int*)key, (const int*)fcs, nf, tot, perm);
+    TLAUNCH("bucket_gather", (mps::k_gather_fluid<real, DIM>), dim3(nblk(nf)), B, 0, stream,
+            (const int*)perm, (const int*)key, nf, tot, fa(0), fa(1),
+            use_cls ? (const uint8_t*)wnear : (const uint8_t*)nullptr, (const int*)fcs, (const int*)nstart,
+            near_list, near_scene);
     swap_all();
     if (S > 1)
       hipLaunchKernelGGL((mps::k_scene_blocks<real>), dim3(1), B, 0, stream, (const mps::Cst<real>*)dtab,
@@ -584,15 +615,25 @@ struct Impl : ImplBase {
     return MPS_OK;
   }
 
-  // ResetMRR + GenMRR_nonslip + MkBkt_MRR (kernel.cu:1906-1908).
-  int mirror() {
+  // ResetMRR + GenMRR_nonslip + MkBkt_MRR (kernel.cu:1906-1908).  `fresh`:
+  // no position has changed since the last bucket(), whose gather left the
+  // near-wall list in near_list / mcount; otherwise it is built here.
+  // Without normal classes mcount is zero here (the general generator counts
+  // from zero; the mirror build's last kernel clears it).
+  int mirror(bool fresh) {
     const dim3 B(mps::kBlock), SB(block), SG(sb_grid);
-    // mcount is zero here: the previous build's rank kernel cleared it
+    if (cnt_dirty) {  // fluid counts no bucket build has used
+      hipLaunchKernelGGL(mps::k_fill_int, dim3(nblk(tot + 1, 4096)), B, 0, stream, cnt, tot + 1, 0);
+      cnt_dirty = keys_ready = false;
+    }
     // raw slots: 10 per listed particle (group path) or compacted (general path)
     int mult = 1;
     if (nf > 0 && use_cls) {
-      TLAUNCH("mirror_near", (mps::k_mirror_near<real, DIM>), SG, SB, 0, stream, fpos[0],
-              (const int*)fcs, sc(), (const uint8_t*)wnear, near_list, near_scene, mcount);
+      if (!fresh) {
+        hipLaunchKernelGGL(mps::k_fill_int, dim3(1), B, 0, stream, mcount, 1, 0);
+        TLAUNCH("mirror_near", (mps::k_mirror_near<real, DIM>), SG, SB, 0, stream, fpos[0],
+                (const int*)fcs, sc(), (const uint8_t*)wnear, near_list, near_scene, mcount);
+      }
       TLAUNCH("mirror_gen", (mps::k_mirror_group<real, DIM>), dim3(std::max(1, std::min(nf, wave_grid))),
               dim3(64), 0, stream, fpos[0], fprs[0], fid[0], near_list, near_scene, mcount,
               (const mps::Cst<real>*)dtab, wpos, wnrm, wcls, wcs, mraw_pos, mraw_prs, mraw_id,
@@ -610,13 +651,12 @@ struct Impl : ImplBase {
     scan(cnt, tot + 1, mcs);
     tock();
     TLAUNCH("mirror_scatter", mps::k_bkt_scatter, dim3(g), B, 0, stream, (const int*)mraw_key, slot,
-            mraw_id, mcs, 0, (const int*)mcount, mult, tmp, tmp_id);
+            mraw_id, mcs, 0, (const int*)mcount, mult, tmp, tmp_id, (int*)nullptr);
     // live mirrors: mcs[tot]
-    TLAUNCH("mirror_rank", mps::k_bkt_rank, dim3(g), B, 0, stream, tmp, tmp_id, (const int*)mraw_key, mcs,
-            0, (const int*)(mcs + tot), tot, perm, mcount);
-    TLAUNCH("mirror_gather", (mps::k_gather_mirror<real, DIM>), dim3(g), B, 0, stream, perm,
-            (const int*)(mcs + tot), mraw_pos, mraw_prs, mraw_id, mpos, mprs, mid, (const int*)mcs,
-            (const mps::Cst<real>*)dtab, S, tot, mnear);
+    TLAUNCH("mirror_gather", (mps::k_rank_gather_mirror<real, DIM>), dim3(g), B, 0, stream, (const int*)tmp,
+            (const int*)tmp_id, (const int*)mraw_key, (const int*)mcs, (const int*)(mcs + tot), perm,
+            (const V*)mraw_pos, (const real*)mraw_prs, (const int*)mraw_id, mpos, mprs, mid,
+            (const mps::Cst<real>*)dtab, S, tot, mnear, mcount);
     return MPS_OK;
   }
 
@@ -633,6 +673,15 @@ struct Impl : ImplBase {
                        (const real*)s_in, s_out, pav);
   }
 
+  // Lanes past the live range carry the deleted tail over (single-scene
+  // handles); `keys`: the kernel moves particles and writes the next bucket
+  // build's keys and counts.
+  mps::Tail tail(bool keys) {
+    if (S != 1) return mps::Tail{};
+    if (keys) keys_ready = cnt_dirty = true;
+    return mps::Tail{nf, tot, keys ? key : nullptr, slot, cnt};
+  }
+
   // d_VscTrm (+ d_UpPcl1 when fuse): fuse writes the new pos/vel and swaps
   // them in; otherwise the acceleration goes to facc.
   void run_vsc(bool fuse) {
@@ -640,14 +689,14 @@ struct Impl : ImplBase {
     if (fuse) {
       TLAUNCH("vsc_up1", (mps::k_vsc<real, DIM, true>), SG, SB, 0, stream, fpos[0], fvel[0],
               (const int*)fcs, sc(), mpos, mcs, (const uint8_t*)mnear, chunks, nf, (V*)nullptr, fpos[1],
-              fvel[1], ftyp[0], fprs[0]);
-      copy_tail(fpos[0], fpos[1], fvel[0], fvel[1], nullptr, nullptr, nullptr);
+              fvel[1], ftyp[0], fprs[0], tail(true));
+      if (S > 1) copy_tail(fpos[0], fpos[1], fvel[0], fvel[1], nullptr, nullptr, nullptr);
       std::swap(fpos[0], fpos[1]);
       std::swap(fvel[0], fvel[1]);
     } else {
       TLAUNCH("vsc", (mps::k_vsc<real, DIM, false>), SG, SB, 0, stream, fpos[0], fvel[0],
               (const int*)fcs, sc(), mpos, mcs, (const uint8_t*)mnear, chunks, nf, facc[0], (V*)nullptr,
-              (V*)nullptr, (int8_t*)nullptr, (real*)nullptr);
+              (V*)nullptr, (int8_t*)nullptr, (real*)nullptr, mps::Tail{});
     }
   }
 
@@ -677,8 +726,8 @@ struct Impl : ImplBase {
   void run_colprs() {
     TLAUNCH("col_prs", (mps::k_colprs_list<real, DIM>), dim3(sb_grid), dim3(block), 0, stream, fpos[0],
             fvel[0], (const int*)fcs, nf, nlist, ncount, wpos, wvel, wcs, mpos, mcs, (const uint8_t*)wnear,
-            (const uint8_t*)mnear, sc(), fvel[1], fprs[0], fnd[0]);
-    copy_tail(fvel[0], fvel[1], nullptr, nullptr, nullptr, nullptr, nullptr);
+            (const uint8_t*)mnear, sc(), fvel[1], fprs[0], fnd[0], tail(false));
+    if (S > 1) copy_tail(fvel[0], fvel[1], nullptr, nullptr, nullptr, nullptr, nullptr);
     std::swap(fvel[0], fvel[1]);
   }
 
@@ -694,30 +743,31 @@ struct Impl : ImplBase {
       if (fuse) {
         TLAUNCH("grad_up2", (mps::k_grad_list<real, DIM, true, true>), SG, SB, 0, stream, fpos[0], fvel[0],
                 fprs[0], (const int*)fcs, nf, nlist, ncount, mpos, mprs, mcs, (const uint8_t*)mnear, sc(),
-                (V*)nullptr, fpos[1], fvel[1], fprs[1], fpav[0], ftyp[0]);
-        copy_tail(fpos[0], fpos[1], fvel[0], fvel[1], fprs[0], fprs[1], fpav[0]);
+                (V*)nullptr, fpos[1], fvel[1], fprs[1], fpav[0], ftyp[0], tail(true));
+        if (S > 1) copy_tail(fpos[0], fpos[1], fvel[0], fvel[1], fprs[0], fprs[1], fpav[0]);
         std::swap(fpos[0], fpos[1]);
         std::swap(fvel[0], fvel[1]);
         std::swap(fprs[0], fprs[1]);
       } else {
         TLAUNCH("grad", (mps::k_grad_list<real, DIM, false, true>), SG, SB, 0, stream, fpos[0], fvel[0],
                 fprs[0], (const int*)fcs, nf, nlist, ncount, mpos, mprs, mcs, (const uint8_t*)mnear, sc(),
-                facc[0], (V*)nullptr, (V*)nullptr, (real*)nullptr, (real*)nullptr, (int8_t*)nullptr);
+                facc[0], (V*)nullptr, (V*)nullptr, (real*)nullptr, (real*)nullptr, (int8_t*)nullptr,
+                mps::Tail{});
       }
       return;
     }
     if (fuse) {
       TLAUNCH("grad_up2", (mps::k_grad_list<real, DIM, true>), SG, SB, 0, stream, fpos[0], fvel[0],
               fprs[0], (const int*)fcs, nf, nlist, ncount, mpos, mprs, mcs, (const uint8_t*)mnear, sc(),
-              (V*)nullptr, fpos[1], fvel[1], fprs[1], fpav[0], ftyp[0]);
-      copy_tail(fpos[0], fpos[1], fvel[0], fvel[1], fprs[0], fprs[1], fpav[0]);
+              (V*)nullptr, fpos[1], fvel[1], fprs[1], fpav[0], ftyp[0], tail(true));
+      if (S > 1) copy_tail(fpos[0], fpos[1], fvel[0], fvel[1], fprs[0], fprs[1], fpav[0]);
       std::swap(fpos[0], fpos[1]);
       std::swap(fvel[0], fvel[1]);
       std::swap(fprs[0], fprs[1]);
     } else {
       TLAUNCH("grad", (mps::k_grad_list<real, DIM, false>), SG, SB, 0, stream, fpos[0], fvel[0], fprs[0],
               (const int*)fcs, nf, nlist, ncount, mpos, mprs, mcs, (const uint8_t*)mnear, sc(), facc[0],
-              (V*)nullptr, (V*)nullptr, (real*)nullptr, (real*)nullptr, (int8_t*)nullptr);
+              (V*)nullptr, (V*)nullptr, (real*)nullptr, (real*)nullptr, (int8_t*)nullptr, mps::Tail{});
     }
   }
 
@@ -733,14 +783,14 @@ struct Impl : ImplBase {
     }
     for (int s = 0; s < nsteps; s++) {
       bucket();
-      mirror();
+      mirror(true);
       if (nf > 0) run_vsc(true);
       bucket();
       if (nf > 0) {
         run_nlist();
         run_colprs();
       }
-      mirror();
+      mirror(true);  // ChkCol changed velocities only
       if (nf > 0) run_grad(true);
       if (wall_pav && nw > 0)
         hipLaunchKernelGGL((mps::k_wall_pav<real>), dim3(nblk(nw)), B, 0, stream, wpav, wprs, nw);
@@ -763,6 +813,10 @@ struct Impl : ImplBase {
 
   int capture() {
     graph_buffers = buffers();
+    // the graph's first bucket build keys and counts by itself: a replay
+    // then does not depend on what ran before it
+    keys_ready = false;
+    cnt_dirty = true;
     HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
     step_eager(graph_steps);
     HIP_TRY(hipStreamEndCapture(stream, &graph));
@@ -782,6 +836,7 @@ struct Impl : ImplBase {
       int rc;
       if (!gexec && (rc = capture())) return rc;
       for (; nsteps >= graph_steps; nsteps -= graph_steps) HIP_TRY(hipGraphLaunch(gexec, stream));
+      keys_ready = cnt_dirty = S == 1;  // as the graph's last step leaves them
     }
     step_eager(nsteps);
     return check_launch();
@@ -817,10 +872,11 @@ struct Impl : ImplBase {
   int phase(int ph) override {
     const dim3 B(mps::kBlock), G(nblk(nf));
     drop_graph();  // host-side buffer order may change
+    keys_ready = false;
     acc_live = true;
     switch (ph) {
       case MPS_PH_MKBKT: bucket(); break;
-      case MPS_PH_MIRROR: mirror(); break;
+      case MPS_PH_MIRROR: mirror(false); break;
       case MPS_PH_VSC:
         if (nf > 0) run_vsc(false);
         break;
@@ -961,6 +1017,7 @@ struct Impl : ImplBase {
       if (typ)
         for (int w = 0; w < nw; w++) wall_typ[w] = typ[host_wall_ids[w]];
     }
+    keys_ready = false;  // positions and types may have changed
     if ((rc = bucket())) return rc;
     HIP_TRY(hipStreamSynchronize(stream));
     return check_launch();
@@ -1122,6 +1179,7 @@ struct Impl : ImplBase {
   int adj_forward(int nsteps, int every) override {
     if (S != 1) return set_err(MPS_E_STATE, "the adjoint runs on single-scene handles");
     if (tensor) return set_err(MPS_E_ARG, "the adjoint supports grad_model MPS_GRAD_STANDARD only");
+    keys_ready = false;
     if (nsteps < 1 || every < 1) return set_err(MPS_E_ARG, "need nsteps >= 1 and checkpoint_every >= 1");
     if (nf == 0) return set_err(MPS_E_STATE, "no fluid particles");
     drop_graph();
@@ -1219,7 +1277,7 @@ struct Impl : ImplBase {
     d2d(T.fcs1, fcs, (size_t)tot + 2);
     d2d(T.x, fpos[0], nf);
     d2d(T.v, fvel[0], nf);
-    mirror();
+    mirror(true);
     tape_mirrors(T.A);
     run_vsc(false);
     d2d(T.a1, facc[0], nf);
@@ -1238,7 +1296,7 @@ struct Impl : ImplBase {
     d2d(T.v2, fvel[0], nf);
     d2d(T.P, fprs[0], nf);
     d2d(T.nd, fnd[0], nf);
-    mirror();
+    mirror(true);
     tape_mirrors(T.B);
     run_grad(false);
     d2d(T.a2, facc[0], nf);
@@ -1308,6 +1366,7 @@ struct Impl : ImplBase {
 
   int adj_backward(const double* xbar, const double* vbar, double* gx0, double* gv0, double* gp) override {
     if (!adj_ready) return set_err(MPS_E_STATE, "mps_adjoint_forward has not run");
+    keys_ready = false;
     drop_graph();
     int rc;
     if (tape.empty() && (rc = alloc_adjoint_work())) return rc;

@@ -13,6 +13,8 @@
 // Heavy kernels run scene-uniform workgroups (scene_block); light ones look
 // up the scene of each fluid entry in `sid`.
 #pragma once
+#include <type_traits>
+
 #include "mps_device.h"
 
 namespace mps {
@@ -144,29 +146,46 @@ __device__ inline void scan_row_staged(V* stage, int b, int e, bool act, const V
 constexpr int kScanIPT = 8;
 constexpr int kScanTile = kBlock * kScanIPT;
 
-__device__ inline int block_exclusive_scan(int v, int* sh, int& total) {
+template <class T>
+__device__ inline T block_exclusive_scan(T v, T* sh, T& total) {
   const int t = threadIdx.x;
   sh[t] = v;
   __syncthreads();
   for (int off = 1; off < kBlock; off <<= 1) {
-    int x = t >= off ? sh[t - off] : 0;
+    T x = t >= off ? sh[t - off] : T(0);
     __syncthreads();
     sh[t] += x;
     __syncthreads();
   }
   total = sh[kBlock - 1];
-  int excl = sh[t] - v;
+  T excl = sh[t] - v;
   __syncthreads();
   return excl;
 }
 
-__global__ __launch_bounds__(kBlock) void k_scan_reduce(const int* in, int n, int* part) {
-  __shared__ int sh[kBlock];
+// Scanned value of count c at index k: c, or with `flag` the pair
+// (c, flag[k] ? c : 0) packed in 64 bits (low, high), so that one scan gives
+// the cell starts and the starts of the flagged cells' entries among the
+// flagged entries.  Indices k >= nflag are unflagged.
+using u64 = unsigned long long;
+template <bool PAIR>
+using ScanT = typename std::conditional<PAIR, u64, int>::type;
+template <bool PAIR>
+__device__ inline ScanT<PAIR> scan_value(int c, const uint8_t* __restrict__ flag, int nflag, int k) {
+  if constexpr (PAIR) return u64(unsigned(c)) | (u64(k < nflag && flag[k] ? unsigned(c) : 0u) << 32);
+  else return c;
+}
+
+template <bool PAIR>
+__global__ __launch_bounds__(kBlock) void k_scan_reduce(const int* in, int n, ScanT<PAIR>* part,
+                                                        const uint8_t* __restrict__ flag, int nflag) {
+  using T = ScanT<PAIR>;
+  __shared__ T sh[kBlock];
   const int base = blockIdx.x * kScanTile + threadIdx.x * kScanIPT;
-  int s = 0;
+  T s = 0;
   for (int k = 0; k < kScanIPT; k++)
-    if (base + k < n) s += in[base + k];
-  int total;
+    if (base + k < n) s += scan_value<PAIR>(in[base + k], flag, nflag, base + k);
+  T total;
   block_exclusive_scan(s, sh, total);
   if (threadIdx.x == 0) part[blockIdx.x] = total;
 }
@@ -175,31 +194,48 @@ __global__ __launch_bounds__(kBlock) void k_scan_reduce(const int* in, int n, in
 // sum of the totals of the tiles before it (part, from k_scan_reduce; null
 // for a single tile), added up by each workgroup itself.  The counts are
 // zeroed once read, so the counting array is clear for the next build.
-__global__ __launch_bounds__(kBlock) void k_scan_down(int* in, int n, const int* part, int* out) {
-  __shared__ int sh[kBlock];
-  int o = 0;
+// PAIR: out gets the low sums, out2 the high ones (flagged entries), and
+// *total2 the high grand total.
+template <bool PAIR>
+__global__ __launch_bounds__(kBlock) void k_scan_down(int* in, int n, const ScanT<PAIR>* part, int* out,
+                                                      const uint8_t* __restrict__ flag, int nflag,
+                                                      int* __restrict__ out2, int* __restrict__ total2) {
+  using T = ScanT<PAIR>;
+  __shared__ T sh[kBlock];
+  T o = 0;
   if (part)
     for (int k = threadIdx.x; k < int(blockIdx.x); k += kBlock) o += part[k];
-  int offset;
+  T offset;
   block_exclusive_scan(o, sh, offset);
   const int base = blockIdx.x * kScanTile + threadIdx.x * kScanIPT;
-  int v[kScanIPT];
-  int s = 0;
+  T v[kScanIPT];
+  T s = 0;
   for (int k = 0; k < kScanIPT; k++) {
     v[k] = 0;
     if (base + k < n) {
-      v[k] = in[base + k];
+      v[k] = scan_value<PAIR>(in[base + k], flag, nflag, base + k);
       in[base + k] = 0;
     }
     s += v[k];
   }
-  int total;
-  int run = block_exclusive_scan(s, sh, total) + offset;
+  T total;
+  T run = block_exclusive_scan(s, sh, total) + offset;
+  auto put = [&](int k, T x) {
+    if constexpr (PAIR) {
+      out[k] = int(unsigned(x));
+      out2[k] = int(unsigned(x >> 32));
+    } else {
+      out[k] = x;
+    }
+  };
   for (int k = 0; k < kScanIPT; k++) {
-    if (base + k < n) out[base + k] = run;
+    if (base + k < n) put(base + k, run);
     run += v[k];
   }
-  if (base <= n && n < base + kScanIPT) out[n] = run;
+  if (base <= n && n < base + kScanIPT) {
+    put(n, run);
+    if constexpr (PAIR) *total2 = int(unsigned(run >> 32));
+  }
 }
 
 // Workgroup table of the scene-uniform launches: blk_off[s] = sum over
@@ -291,6 +327,30 @@ __device__ inline int wave_count(int* cnt, int k) {
   return slot;
 }
 
+// Work of the lanes past the live range in the fused kernels of a
+// single-scene handle, whose grid then covers the whole fluid set: the
+// deleted tail [live end, n) is carried over to the output buffers, and the
+// kernels that move particles write the next bucket build's key (tot for a
+// deleted particle) and slot as k_bkt_key would, for every entry.  n = 0:
+// none of this (the host copies the tail, the bucket build keys).
+struct Tail {
+  int n, tot;
+  int* key;  // null: no keys
+  int* slot;
+  int* cnt;
+};
+
+template <class real, int DIM>
+__device__ inline void next_key(const Tail& tl, int i, bool live, const Vec<real, DIM>& p, const Cst<real>& c) {
+  int k = tl.tot;
+  if (live) {
+    const Cell<real, DIM> q = cell_of<real, DIM>(p, c);
+    k = cell_key(q.ix, q.iy, q.iz, c);
+  }
+  tl.key[i] = k;
+  tl.slot[i] = wave_count(tl.cnt, k);
+}
+
 template <class real, int DIM>
 __global__ void k_bkt_key(const Vec<real, DIM>* __restrict__ pos, const int8_t* __restrict__ typ,
                           const int* __restrict__ sid, const Cst<real>* __restrict__ tab, int n,
@@ -319,11 +379,13 @@ __global__ void k_key_count(const int* __restrict__ key, const int* __restrict__
 }
 
 // Element count: n_static, or mult * (*n_dyn) when n_dyn is given.  The
-// element's id goes beside it (tmp_id) for the rank kernel.
+// element's id goes beside it (tmp_id) for the rank kernel.  `clear` (may be
+// null, and not n_dyn) is zeroed.
 __global__ void k_bkt_scatter(const int* __restrict__ key, const int* __restrict__ slot,
                               const int* __restrict__ id, const int* __restrict__ cs, int n_static,
                               const int* __restrict__ n_dyn, int mult, int* __restrict__ tmp,
-                              int* __restrict__ tmp_id) {
+                              int* __restrict__ tmp_id, int* __restrict__ clear) {
+  if (clear && blockIdx.x == 0 && threadIdx.x == 0) *clear = 0;
   const int n = n_dyn ? *n_dyn * mult : n_static;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const int k = key[i];
@@ -340,32 +402,19 @@ __global__ void k_bkt_scatter(const int* __restrict__ key, const int* __restrict
 // from tmp_id; past the cell's end the first id is read and not counted).
 // The deleted tail (key == tot) keeps the scatter order: its order is never
 // observed.
-// `clear` (may be null), no longer read once the scatter has run, is zeroed.
-__global__ void k_bkt_rank(const int* __restrict__ tmp, const int* __restrict__ tmp_id,
-                           const int* __restrict__ key, const int* __restrict__ cs, int n_static,
-                           const int* __restrict__ n_dyn, int tot, int* __restrict__ perm,
-                           int* __restrict__ clear) {
-  if (clear && blockIdx.x == 0 && threadIdx.x == 0) *clear = 0;
-  const int n = n_dyn ? *n_dyn : n_static;
-  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
-    const int i = tmp[s];
-    const int k = key[i];
-    if (k == tot) {
-      perm[s] = i;
-      continue;
-    }
-    const int b = cs[k], e = cs[k + 1];
-    const int me = tmp_id[s];
-    int rank = 0;
-    for (int t = b; t < e; t += 8) {
-      int v[8];
+__device__ inline int bucket_rank(const int* __restrict__ tmp_id, const int* __restrict__ cs, int k,
+                                  int s) {
+  const int b = cs[k], e = cs[k + 1];
+  const int me = tmp_id[s];
+  int rank = 0;
+  for (int t = b; t < e; t += 8) {
+    int v[8];
 #pragma unroll
-      for (int u = 0; u < 8; u++) v[u] = tmp_id[t + u < e ? t + u : b];
+    for (int u = 0; u < 8; u++) v[u] = tmp_id[t + u < e ? t + u : b];
 #pragma unroll
-      for (int u = 0; u < 8; u++) rank += (t + u < e) & (v[u] < me);
-    }
-    perm[b + rank] = i;
+    for (int u = 0; u < 8; u++) rank += (t + u < e) & (v[u] < me);
   }
+  return b + rank;
 }
 
 template <class real, int DIM>
@@ -381,9 +430,30 @@ struct FluidArrays {
   int* sid;
 };
 
+// perm[final place] = scatter-order element (see bucket_rank).  The deleted
+// tail (key == tot) keeps the scatter order.  One pass over the fluid set;
+// the gather below then reads through perm and writes in order.
+__global__ void k_bkt_rank(const int* __restrict__ tmp, const int* __restrict__ tmp_id,
+                           const int* __restrict__ key, const int* __restrict__ cs, int n, int tot,
+                           int* __restrict__ perm) {
+  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+    const int i = tmp[s];
+    const int k = key[i];
+    perm[k == tot ? s : bucket_rank(tmp_id, cs, k, s)] = i;
+  }
+}
+
+// dst[s] = src[perm[s]] for every fluid field.  With wnear it also lists the
+// live particles whose bucket stencil holds a wall, in sorted order: the
+// particle at s in bucket k goes to near_list[nstart[k] + s - cs[k]]
+// (nstart: the bucket scan's starts among near-wall particles), its scene
+// beside it in near_scene.  The mirror generator's work list.
 template <class real, int DIM>
-__global__ void k_gather_fluid(const int* __restrict__ perm, int n, FluidArrays<real, DIM> src,
-                               FluidArrays<real, DIM> dst) {
+__global__ void k_gather_fluid(const int* __restrict__ perm, const int* __restrict__ key, int n, int tot,
+                               FluidArrays<real, DIM> src, FluidArrays<real, DIM> dst,
+                               const uint8_t* __restrict__ wnear, const int* __restrict__ cs,
+                               const int* __restrict__ nstart, int* __restrict__ near_list,
+                               int* __restrict__ near_scene) {
   for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
     const int i = perm[s];
     dst.pos[s] = src.pos[i];
@@ -394,26 +464,41 @@ __global__ void k_gather_fluid(const int* __restrict__ perm, int n, FluidArrays<
     dst.nd[s] = src.nd[i];
     dst.typ[s] = src.typ[i];
     dst.id[s] = src.id[i];
-    dst.sid[s] = src.sid[i];
+    const int sid = src.sid[i];
+    dst.sid[s] = sid;
+    if (wnear) {
+      const int k = key[i];
+      if (k != tot && wnear[k]) {
+        const int t = nstart[k] + (s - cs[k]);
+        near_list[t] = s;
+        near_scene[t] = sid;
+      }
+    }
   }
 }
 
-// Also sets near[k] (any mirror in the stencil of bucket k) from the new
-// cell starts, as k_cell_near does.
+// Rank + gather of the live mirrors (count *n_dyn = the mirror cell_start's
+// total); also sets near[k] (any mirror in the stencil of bucket k) from the
+// new cell starts, as k_cell_near does.  `clear` (may be null), no longer
+// read once the scatter has run, is zeroed.
 template <class real, int DIM>
-__global__ void k_gather_mirror(const int* __restrict__ perm, const int* __restrict__ n_dyn,
-                                const Vec<real, DIM>* __restrict__ spos,
-                                const real* __restrict__ sprs, const int* __restrict__ sid,
-                                Vec<real, DIM>* __restrict__ dpos, real* __restrict__ dprs,
-                                int* __restrict__ did, const int* __restrict__ cs,
-                                const Cst<real>* __restrict__ tab, int S, int tot,
-                                uint8_t* __restrict__ near) {
+__global__ void k_rank_gather_mirror(const int* __restrict__ tmp, const int* __restrict__ tmp_id,
+                                     const int* __restrict__ key, const int* __restrict__ cs,
+                                     const int* __restrict__ n_dyn, int* __restrict__ perm,
+                                     const Vec<real, DIM>* __restrict__ spos, const real* __restrict__ sprs,
+                                     const int* __restrict__ sid, Vec<real, DIM>* __restrict__ dpos,
+                                     real* __restrict__ dprs, int* __restrict__ did,
+                                     const Cst<real>* __restrict__ tab, int S, int tot,
+                                     uint8_t* __restrict__ near, int* __restrict__ clear) {
+  if (clear && blockIdx.x == 0 && threadIdx.x == 0) *clear = 0;
   const int n = *n_dyn;
   for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
-    const int i = perm[s];
-    dpos[s] = spos[i];
-    dprs[s] = sprs[i];
-    did[s] = sid[i];
+    const int i = tmp[s];
+    const int d = bucket_rank(tmp_id, cs, key[i], s);
+    perm[d] = i;
+    dpos[d] = spos[i];
+    dprs[d] = sprs[i];
+    did[d] = sid[i];
   }
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < tot; k += gridDim.x * blockDim.x)
     near[k] = cell_near<real, DIM>(cs, tab, S, k);
@@ -967,17 +1052,29 @@ __device__ inline void walk_chunks(const Chunk* buf, size_t n_alloc, int i, int 
 // terms in that order: fluid neighbours by stencil order, then mirrors,
 // as the reference's two loops do.
 template <class real, int DIM, bool FUSE>
-__global__ __launch_bounds__(kBlock) void k_vsc(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) void k_vsc(
     const Vec<real, DIM>* __restrict__ pos, const Vec<real, DIM>* __restrict__ vel,
     const int* __restrict__ fcs, Scenes<real> sc, const Vec<real, DIM>* __restrict__ mpos,
     const int* __restrict__ mcs, const uint8_t* __restrict__ mnear, Chunk* chunks, int n_alloc,
     Vec<real, DIM>* __restrict__ acc_out, Vec<real, DIM>* __restrict__ pos_out,
-    Vec<real, DIM>* __restrict__ vel_out, int8_t* __restrict__ typ, real* __restrict__ prs) {
+    Vec<real, DIM>* __restrict__ vel_out, int8_t* __restrict__ typ, real* __restrict__ prs, Tail tl) {
   using V = Vec<real, DIM>;
   __shared__ V s_stage[kBlock / 64][stage_cap<V>() + 8];
   int s, first, end;
   Cst<real> c;
-  if (!scene_block(sc, fcs, s, c, first, end)) return;  // whole workgroup
+  auto tail = [&](int t) {  // a lane past the live range (FUSE)
+    if (t < end || t >= tl.n) return;
+    pos_out[t] = pos[t];
+    vel_out[t] = vel[t];
+    if (tl.key) {
+      tl.key[t] = tl.tot;
+      tl.slot[t] = wave_count(tl.cnt, tl.tot);
+    }
+  };
+  if (!scene_block(sc, fcs, s, c, first, end)) {  // whole workgroup
+    if constexpr (FUSE) tail(first + int(threadIdx.x));
+    return;
+  }
   // every lane stays for the wave-wide staging; act marks a particle
   const bool act = first + int(threadIdx.x) < end;
   const int i = act ? first + int(threadIdx.x) : first;
@@ -1047,7 +1144,10 @@ __global__ __launch_bounds__(kBlock) void k_vsc(
             [&](const V& pj) { return near2(pj.x - px, pj.y - py, vz(pj) - pz) <= r2_hi; },
             [&](int base, unsigned long long m) { emit(base | kMirrorChunk, m); });
       }
-  if (!act) return;
+  if (!act) {
+    if constexpr (FUSE) tail(first + int(threadIdx.x));
+    return;
+  }
   walk();
   const real accx = c.n0knm * ax + c.g[0];
   const real accy = c.n0knm * ay + c.g[1];
@@ -1057,12 +1157,15 @@ __global__ __launch_bounds__(kBlock) void k_vsc(
   } else {
     real nvx = vx + accx * c.dt, nvy = vy + accy * c.dt, nvz = DIM == 3 ? vvz + accz * c.dt : real(0);
     real nx = px + nvx * c.dt, ny = py + nvy * c.dt, nz = DIM == 3 ? pz + nvz * c.dt : real(0);
-    if (!chk_pcl<real, DIM>(nx, ny, nz, nvx, nvy, nvz, c)) {
+    const bool live = chk_pcl<real, DIM>(nx, ny, nz, nvx, nvy, nvz, c);
+    if (!live) {
       typ[i] = kGST;
       prs[i] = real(0);
     }
-    pos_out[i] = mkvec<real, DIM>(nx, ny, nz);
+    const Vec<real, DIM> np = mkvec<real, DIM>(nx, ny, nz);
+    pos_out[i] = np;
     vel_out[i] = mkvec<real, DIM>(nvx, nvy, nvz);
+    if (tl.key) next_key(tl, i, live, np, c);
   }
 }
 

@@ -162,12 +162,18 @@ __global__ __launch_bounds__(kBlock) void k_colprs_list(
     const Vec<real, DIM>* __restrict__ wvel, const int* __restrict__ wcs,
     const Vec<real, DIM>* __restrict__ mpos, const int* __restrict__ mcs,
     const uint8_t* __restrict__ wnear, const uint8_t* __restrict__ mnear, Scenes<real> sc,
-    Vec<real, DIM>* __restrict__ vel_out, real* __restrict__ prs, real* __restrict__ nd) {
+    Vec<real, DIM>* __restrict__ vel_out, real* __restrict__ prs, real* __restrict__ nd, Tail tl) {
   using V = Vec<real, DIM>;
   __shared__ V s_stage[kBlock / 64][stage_cap<V>() + 8];
   int s, first, end;
   Cst<real> c;
-  if (!scene_block(sc, fcs, s, c, first, end)) return;  // whole workgroup
+  auto tail = [&](int t) {  // a lane past the live range
+    if (t >= end && t < tl.n) vel_out[t] = vel[t];
+  };
+  if (!scene_block(sc, fcs, s, c, first, end)) {  // whole workgroup
+    tail(first + int(threadIdx.x));
+    return;
+  }
   // every lane stays for the wave-wide staging; act marks a particle
   const bool act = first + int(threadIdx.x) < end;
   const int i = act ? first + int(threadIdx.x) : first;
@@ -260,7 +266,10 @@ __global__ __launch_bounds__(kBlock) void k_colprs_list(
       }
       if (d2 < rlim2) apply(qx, qy, qz, d2, mkvec<real, DIM>(0, 0, 0));
     });
-  if (!act) return;
+  if (!act) {
+    tail(first + int(threadIdx.x));
+    return;
+  }
   vel_out[i] = mkvec<real, DIM>(v2x, v2y, v2z);
   real pressure = real(ni > c.n0_grad) * (ni - c.n0_grad) * c.prs_coef * c.mass;
   if (pressure <= c.pmax && pressure >= real(0)) {
@@ -410,12 +419,26 @@ __global__ __launch_bounds__(kBlock) void k_grad_list(
     const int* __restrict__ mcs, const uint8_t* __restrict__ mnear, Scenes<real> sc,
     Vec<real, DIM>* __restrict__ acc_out, Vec<real, DIM>* __restrict__ pos_out,
     Vec<real, DIM>* __restrict__ vel_out, real* __restrict__ prs_out, real* __restrict__ pav,
-    int8_t* __restrict__ typ) {
+    int8_t* __restrict__ typ, Tail tl) {
   using V = Vec<real, DIM>;
   __shared__ V s_stage[kBlock / 64][stage_cap<V>() + 8];
   int s, first, end;
   Cst<real> c;
-  if (!scene_block(sc, fcs, s, c, first, end)) return;  // whole workgroup
+  auto tail = [&](int t) {  // a lane past the live range (FUSE)
+    if (t < end || t >= tl.n) return;
+    pos_out[t] = pos[t];
+    vel_out[t] = vel[t];
+    prs_out[t] = prs[t];
+    pav[t] += prs[t];
+    if (tl.key) {
+      tl.key[t] = tl.tot;
+      tl.slot[t] = wave_count(tl.cnt, tl.tot);
+    }
+  };
+  if (!scene_block(sc, fcs, s, c, first, end)) {  // whole workgroup
+    if constexpr (FUSE) tail(first + int(threadIdx.x));
+    return;
+  }
   // every lane stays for the wave-wide staging; act marks a particle
   const bool act = first + int(threadIdx.x) < end;
   const int i = act ? first + int(threadIdx.x) : first;
@@ -473,7 +496,10 @@ __global__ __launch_bounds__(kBlock) void k_grad_list(
       [&](int, const Vec<real, DIM>& pj, real pr) { add(pj, pr); }, prs);
   for_mirrors_near<real, DIM>(stage, act && mnear[cell_key(q.ix, q.iy, q.iz, c)], px, py, pz, mpos,
                               mcs, q, c, [&](int j, const Vec<real, DIM>& pj) { add(pj, mprs[j]); });
-  if (!act) return;
+  if (!act) {
+    if constexpr (FUSE) tail(first + int(threadIdx.x));
+    return;
+  }
   real accx, accy, accz;
   if constexpr (TENSOR) {
     const real invn0 = real(1) / c.n0_grad;
@@ -528,13 +554,16 @@ __global__ __launch_bounds__(kBlock) void k_grad_list(
     real nz = DIM == 3 ? pz + accz * c.dt * c.dt : real(0);
     pav[i] += prs_i;
     real po = prs_i;
-    if (!chk_pcl<real, DIM>(nx, ny, nz, nvx, nvy, nvz, c)) {
+    const bool live = chk_pcl<real, DIM>(nx, ny, nz, nvx, nvy, nvz, c);
+    if (!live) {
       typ[i] = kGST;
       po = real(0);
     }
-    pos_out[i] = mkvec<real, DIM>(nx, ny, nz);
+    const Vec<real, DIM> np = mkvec<real, DIM>(nx, ny, nz);
+    pos_out[i] = np;
     vel_out[i] = mkvec<real, DIM>(nvx, nvy, nvz);
     prs_out[i] = po;
+    if (tl.key) next_key(tl, i, live, np, c);
   }
 }
 
