@@ -637,7 +637,7 @@ struct Impl : ImplBase {
       TLAUNCH("mirror_gen", (mps::k_mirror_group<real, DIM>), dim3(std::max(1, std::min(nf, wave_grid))),
               dim3(64), 0, stream, fpos[0], fprs[0], fid[0], near_list, near_scene, mcount,
               (const mps::Cst<real>*)dtab, wpos, wnrm, wcls, wcs, mraw_pos, mraw_prs, mraw_id,
-              mraw_key);
+              mraw_key, slot, cnt);  // counts the keys too
       mult = mps::kNumMRR;
     } else if (nf > 0) {
       TLAUNCH("mirror_gen", (mps::k_mirror_gen<real, DIM>), SG, SB, 0, stream, fpos[0], fprs[0], fid[0],
@@ -645,8 +645,9 @@ struct Impl : ImplBase {
               mcount, mcap);
     }
     const int g = nblk(mcap, 2048);
-    TLAUNCH("mirror_key", mps::k_key_count, dim3(g), B, 0, stream, (const int*)mraw_key,
-            (const int*)mcount, mult, slot, cnt);
+    if (!(nf > 0 && use_cls))
+      TLAUNCH("mirror_key", mps::k_key_count, dim3(g), B, 0, stream, (const int*)mraw_key,
+              (const int*)mcount, mult, slot, cnt);
     tick("mirror_scan");
     scan(cnt, tot + 1, mcs);
     tock();

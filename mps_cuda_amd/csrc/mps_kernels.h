@@ -817,7 +817,7 @@ __global__ __launch_bounds__(64) void k_mirror_group(
     const Cst<real>* __restrict__ tab, const Vec<real, DIM>* __restrict__ wpos,
     const Vec<real, DIM>* __restrict__ wnrm, const uint8_t* __restrict__ wcls,
     const int* __restrict__ wcs, Vec<real, DIM>* __restrict__ mpos, real* __restrict__ mprs,
-    int* __restrict__ mid, int* __restrict__ mkey) {
+    int* __restrict__ mid, int* __restrict__ mkey, int* __restrict__ mslot, int* __restrict__ cnt) {
   constexpr int P = 64 / kMG;
   __shared__ int first_reg[P][64], first_edge[P][64], sorted_reg[P][kNumMRR], sorted_edge[P][kNumMRR];
   __shared__ Vec<real, DIM> s_pos[P][kNumMRR];
@@ -902,11 +902,13 @@ __global__ __launch_bounds__(64) void k_mirror_group(
     __syncthreads();
     if (act && gl < kNumMRR) {
       const size_t o = (size_t)t * kNumMRR + gl;
-      mkey[o] = s_key[g][gl];
-      if (s_key[g][gl] >= 0) {
+      const int k = s_key[g][gl];
+      mkey[o] = k;
+      if (k >= 0) {
         mpos[o] = s_pos[g][gl];
         mprs[o] = s_prs[g][gl];
         mid[o] = s_id[g][gl];
+        mslot[o] = atomicAdd(&cnt[k], 1);  // the mirror bucket build's count (as k_key_count)
       }
     }
     __syncthreads();
