@@ -110,6 +110,27 @@ def test_fused_steps_bit_exact(oracle_mod, name, make, steps):
         assert_mirrors_equal(g, o, where)
 
 
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_steps_and_phases_interleaved_bit_exact(oracle_mod, prec):
+    """Fused steps, then one step as separate phases, then fused steps again:
+    the fused kernels' bucket keys and counts must not leak into the phase
+    calls, nor the phase calls' state into the next fused step."""
+    scene, params = configs.small3d(prec)
+    g, o = _pair(oracle_mod, scene, params)
+    for h in (g, o):
+        h.step(3)
+    for k, ph in enumerate(ffi.STEP_PHASES):
+        g.phase(ph)
+        o.phase(ph)
+        assert_states_equal(g.state(), o.state(), f"{prec} phase#{k} ({ph}) after 3 steps")
+    for h in (g, o):
+        h.step(4)
+    assert_states_equal(g.state(), o.state(), f"{prec} 3 steps + phases + 4 steps",
+                        fields=("typ", "pos", "vel", "prs", "pav", "nd"))
+    assert_buckets_equal(g, o, ffi.BKT_FLUID, f"{prec} interleaved")
+    assert_mirrors_equal(g, o, f"{prec} interleaved")
+
+
 @pytest.mark.parametrize("mode", [ffi.PMAX_NONE, ffi.PMAX_REFERENCE])
 def test_pressure_clamp_modes_bit_exact(oracle_mod, mode):
     scene, params = configs.cfg0("fp64", pmax_mode=mode)
