@@ -66,6 +66,9 @@ struct ImplBase {
   virtual int get_edge(int8_t* edge) = 0;
   hipStream_t stream = nullptr;
   hipStream_t own_stream = nullptr;
+  // second stream of the step (default priority) and its fork / join events
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   std::vector<void*> allocs;
   int alloc_raw(void** p, size_t bytes) {
     if (bytes == 0) bytes = 256;
@@ -91,6 +94,11 @@ struct ImplBase {
     pool.clear();
     if (own_stream) (void)hipStreamDestroy(own_stream);
     own_stream = nullptr;
+    if (side) (void)hipStreamDestroy(side);
+    side = nullptr;
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
+    ev_fork = ev_join = nullptr;
   }
 
   // ---- optional per-kernel timing with HIP events on the handle's stream ----
@@ -209,8 +217,18 @@ struct Impl : ImplBase {
   real *wprs = nullptr, *wpav = nullptr;
   // mirrors
   V *mraw_pos = nullptr, *mpos = nullptr;
-  real *mraw_prs = nullptr, *mprs = nullptr;
+  real* mprs = nullptr;
   int *mraw_id = nullptr, *mraw_key = nullptr, *mid = nullptr, *mcs = nullptr, *mcount = nullptr;
+  int *mraw_par = nullptr, *mpar = nullptr;  // a mirror's parent: its place in the fluid set
+  // The mirror set the kernels read is (mpos, mprs, mid, mcs, mnear, mpar);
+  // the second set lets the step build the next one while the current one
+  // is still read (swap_mirrors).
+  V* mpos2 = nullptr;
+  real* mprs2 = nullptr;
+  int *mid2 = nullptr, *mcs2 = nullptr, *mpar2 = nullptr;
+  uint8_t* mnear2 = nullptr;
+  int mswaps = 0;  // parity of swap_mirrors calls
+  bool capturing = false;  // a graph is being recorded: one stream only
   int *near_list = nullptr, *near_scene = nullptr;  // fluid particles with walls in their stencil
   int *nlist = nullptr, *ncount = nullptr;          // ordered fluid neighbours within rp
   mps::Chunk* chunks = nullptr;                     // per-particle hit records of VscTrm's scan
@@ -356,8 +374,11 @@ struct Impl : ImplBase {
         (rc = alloc(&wedge, nw)) || (rc = alloc(&wcls, nw)) || (rc = alloc(&wprs, nw)) ||
         (rc = alloc(&wpav, nw)) || (rc = alloc(&wcs, (size_t)tot + 1)))
       return rc;
-    if ((rc = alloc(&mraw_pos, mcap)) || (rc = alloc(&mpos, mcap)) || (rc = alloc(&mraw_prs, mcap)) ||
+    if ((rc = alloc(&mraw_pos, mcap)) || (rc = alloc(&mpos, mcap)) || (rc = alloc(&mraw_par, mcap)) ||
         (rc = alloc(&mprs, mcap)) || (rc = alloc(&mraw_id, mcap)) || (rc = alloc(&mraw_key, mcap)) ||
+        (rc = alloc(&mpar, mcap)) || (rc = alloc(&mpos2, mcap)) || (rc = alloc(&mprs2, mcap)) ||
+        (rc = alloc(&mid2, mcap)) || (rc = alloc(&mpar2, mcap)) || (rc = alloc(&mcs2, (size_t)tot + 2)) ||
+        (rc = alloc(&mnear2, std::max(tot, 1))) ||
         (rc = alloc(&mid, mcap)) || (rc = alloc(&mcs, (size_t)tot + 2)) || (rc = alloc(&mcount, 1)) ||
         (rc = alloc(&near_list, std::max(nf, 1))) || (rc = alloc(&near_scene, std::max(nf, 1))) ||
         (rc = alloc(&nlist, (size_t)mps::kNList * std::max(nf, 1))) ||
@@ -465,11 +486,17 @@ struct Impl : ImplBase {
     hipLaunchKernelGGL((mps::k_cell_near<real, DIM>), dim3(nblk(tot)), B, 0, stream, (const int*)wcs,
                        (const mps::Cst<real>*)dtab, S, tot, wnear);
     hipLaunchKernelGGL(mps::k_fill_int, dim3(nblk(tot + 2, 4096)), B, 0, stream, mcs, tot + 2, 0);
+    hipLaunchKernelGGL(mps::k_fill_int, dim3(nblk(tot + 2, 4096)), B, 0, stream, mcs2, tot + 2, 0);
     hipLaunchKernelGGL(mps::k_fill_int, dim3(1), B, 0, stream, mcount, 1, 0);
     // the counting array stays zero between builds (the scan clears it)
     hipLaunchKernelGGL(mps::k_fill_int, dim3(nblk(tot + 1, 4096)), B, 0, stream, cnt, tot + 1, 0);
     hipLaunchKernelGGL((mps::k_fill<uint8_t>), dim3(nblk(tot, 4096)), B, 0, stream, mnear, tot,
                        (uint8_t)0);
+    hipLaunchKernelGGL((mps::k_fill<uint8_t>), dim3(nblk(tot, 4096)), B, 0, stream, mnear2, tot,
+                       (uint8_t)0);
+    if (!side) HIP_TRY(hipStreamCreate(&side));
+    if (!ev_fork) HIP_TRY(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+    if (!ev_join) HIP_TRY(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
     if ((rc = bucket())) return rc;  // ClcMPS starts with MkBkt (kernel.cu:1878)
     HIP_TRY(hipStreamSynchronize(stream));
     return check_launch();
@@ -620,7 +647,9 @@ struct Impl : ImplBase {
   // near-wall list in near_list / mcount; otherwise it is built here.
   // Without normal classes mcount is zero here (the general generator counts
   // from zero; the mirror build's last kernel clears it).
-  int mirror(bool fresh) {
+  // with_prs: the mirror pressures are taken now; otherwise k_mirror_prs
+  // fills them later through mpar.
+  int mirror(bool fresh, bool with_prs = true) {
     const dim3 B(mps::kBlock), SB(block), SG(sb_grid);
     if (cnt_dirty) {  // fluid counts no bucket build has used
       hipLaunchKernelGGL(mps::k_fill_int, dim3(nblk(tot + 1, 4096)), B, 0, stream, cnt, tot + 1, 0);
@@ -635,13 +664,13 @@ struct Impl : ImplBase {
                 (const int*)fcs, sc(), (const uint8_t*)wnear, near_list, near_scene, mcount);
       }
       TLAUNCH("mirror_gen", (mps::k_mirror_group<real, DIM>), dim3(std::max(1, std::min(nf, wave_grid))),
-              dim3(64), 0, stream, fpos[0], fprs[0], fid[0], near_list, near_scene, mcount,
-              (const mps::Cst<real>*)dtab, wpos, wnrm, wcls, wcs, mraw_pos, mraw_prs, mraw_id,
+              dim3(64), 0, stream, fpos[0], fid[0], near_list, near_scene, mcount,
+              (const mps::Cst<real>*)dtab, wpos, wnrm, wcls, wcs, mraw_pos, mraw_par, mraw_id,
               mraw_key, slot, cnt);  // counts the keys too
       mult = mps::kNumMRR;
     } else if (nf > 0) {
-      TLAUNCH("mirror_gen", (mps::k_mirror_gen<real, DIM>), SG, SB, 0, stream, fpos[0], fprs[0], fid[0],
-              (const int*)fcs, sc(), wpos, wnrm, wedge, wcs, mraw_pos, mraw_prs, mraw_id, mraw_key,
+      TLAUNCH("mirror_gen", (mps::k_mirror_gen<real, DIM>), SG, SB, 0, stream, fpos[0], fid[0],
+              (const int*)fcs, sc(), wpos, wnrm, wedge, wcs, mraw_pos, mraw_par, mraw_id, mraw_key,
               mcount, mcap);
     }
     const int g = nblk(mcap, 2048);
@@ -656,9 +685,20 @@ struct Impl : ImplBase {
     // live mirrors: mcs[tot]
     TLAUNCH("mirror_gather", (mps::k_rank_gather_mirror<real, DIM>), dim3(g), B, 0, stream, (const int*)tmp,
             (const int*)tmp_id, (const int*)mraw_key, (const int*)mcs, (const int*)(mcs + tot), perm,
-            (const V*)mraw_pos, (const real*)mraw_prs, (const int*)mraw_id, mpos, mprs, mid,
+            (const V*)mraw_pos, (const int*)mraw_par, (const int*)mraw_id,
+            with_prs ? (const real*)fprs[0] : (const real*)nullptr, mpos, mpar, mprs, mid,
             (const mps::Cst<real>*)dtab, S, tot, mnear, mcount);
     return MPS_OK;
+  }
+
+  void swap_mirrors() {
+    std::swap(mpos, mpos2);
+    std::swap(mprs, mprs2);
+    std::swap(mid, mid2);
+    std::swap(mcs, mcs2);
+    std::swap(mnear, mnear2);
+    std::swap(mpar, mpar2);
+    mswaps ^= 1;
   }
 
   int check_launch() {
@@ -787,11 +827,35 @@ struct Impl : ImplBase {
       mirror(true);
       if (nf > 0) run_vsc(true);
       bucket();
+      // GenMRR after MkPrs (kernel.cu:1924-1927) uses the positions of this
+      // bucket build (ChkCol changes velocities only) and the new pressures:
+      // the mirror set is built on the side stream into the second set while
+      // ChkCol + MkPrs read the current one; the pressures are filled in
+      // once MkPrs is done.  With graphs on, every step keeps one stream
+      // (a captured step must leave the mirror sets where it found them).
+      const bool overlap = nf > 0 && use_cls && !capturing && graph_steps == 0;
+      if (overlap) {
+        (void)hipEventRecord(ev_fork, stream);
+        (void)hipStreamWaitEvent(side, ev_fork, 0);
+        swap_mirrors();
+        std::swap(stream, side);
+        mirror(true, false);
+        std::swap(stream, side);
+        swap_mirrors();
+        (void)hipEventRecord(ev_join, side);
+      }
       if (nf > 0) {
         run_nlist();
         run_colprs();
       }
-      mirror(true);  // ChkCol changed velocities only
+      if (overlap) {
+        (void)hipStreamWaitEvent(stream, ev_join, 0);
+        swap_mirrors();
+        hipLaunchKernelGGL((mps::k_mirror_prs<real>), dim3(nblk(mcap, 2048)), B, 0, stream,
+                           (const int*)(mcs + tot), (const int*)mpar, (const real*)fprs[0], mprs);
+      } else {
+        mirror(true);  // ChkCol changed velocities only
+      }
       if (nf > 0) run_grad(true);
       if (wall_pav && nw > 0)
         hipLaunchKernelGGL((mps::k_wall_pav<real>), dim3(nblk(nw)), B, 0, stream, wpav, wprs, nw);
@@ -808,7 +872,7 @@ struct Impl : ImplBase {
   // Captures graph_steps steps.  An even count returns every double-buffered
   // field to the buffer it started in, so the graph can be replayed.
   std::vector<const void*> buffers() const {
-    return {fpos[0], fvel[0], facc[0], fprs[0], fpav[0], fnd[0], ftyp[0], fid[0], fsid[0]};
+    return {fpos[0], fvel[0], facc[0], fprs[0], fpav[0], fnd[0], ftyp[0], fid[0], fsid[0], mpos};
   }
   std::vector<const void*> graph_buffers;  // buffer order the graph was captured with
 
@@ -819,7 +883,9 @@ struct Impl : ImplBase {
     keys_ready = false;
     cnt_dirty = true;
     HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    capturing = true;
     step_eager(graph_steps);
+    capturing = false;
     HIP_TRY(hipStreamEndCapture(stream, &graph));
     if (buffers() != graph_buffers) {
       drop_graph();

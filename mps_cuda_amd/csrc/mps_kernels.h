@@ -479,14 +479,17 @@ __global__ void k_gather_fluid(const int* __restrict__ perm, const int* __restri
 
 // Rank + gather of the live mirrors (count *n_dyn = the mirror cell_start's
 // total); also sets near[k] (any mirror in the stencil of bucket k) from the
-// new cell starts, as k_cell_near does.  `clear` (may be null), no longer
-// read once the scatter has run, is zeroed.
+// new cell starts, as k_cell_near does.  A mirror's pressure is its parent's
+// (kernel.cu:833): taken from fprs through the parent's place, or, with
+// fprs null, left to k_mirror_prs.  `clear` (may be null), no longer read
+// once the scatter has run, is zeroed.
 template <class real, int DIM>
 __global__ void k_rank_gather_mirror(const int* __restrict__ tmp, const int* __restrict__ tmp_id,
                                      const int* __restrict__ key, const int* __restrict__ cs,
                                      const int* __restrict__ n_dyn, int* __restrict__ perm,
-                                     const Vec<real, DIM>* __restrict__ spos, const real* __restrict__ sprs,
-                                     const int* __restrict__ sid, Vec<real, DIM>* __restrict__ dpos,
+                                     const Vec<real, DIM>* __restrict__ spos, const int* __restrict__ spar,
+                                     const int* __restrict__ sid, const real* __restrict__ fprs,
+                                     Vec<real, DIM>* __restrict__ dpos, int* __restrict__ dpar,
                                      real* __restrict__ dprs, int* __restrict__ did,
                                      const Cst<real>* __restrict__ tab, int S, int tot,
                                      uint8_t* __restrict__ near, int* __restrict__ clear) {
@@ -497,11 +500,23 @@ __global__ void k_rank_gather_mirror(const int* __restrict__ tmp, const int* __r
     const int d = bucket_rank(tmp_id, cs, key[i], s);
     perm[d] = i;
     dpos[d] = spos[i];
-    dprs[d] = sprs[i];
+    const int par = spar[i];
+    dpar[d] = par;
+    if (fprs) dprs[d] = fprs[par];
     did[d] = sid[i];
   }
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < tot; k += gridDim.x * blockDim.x)
     near[k] = cell_near<real, DIM>(cs, tab, S, k);
+}
+
+// Mirror pressure = the parent's current pressure (kernel.cu:833), for a
+// mirror set generated before that pressure was computed.
+template <class real>
+__global__ void k_mirror_prs(const int* __restrict__ n_dyn, const int* __restrict__ par,
+                             const real* __restrict__ fprs, real* __restrict__ mprs) {
+  const int n = *n_dyn;
+  for (int m = blockIdx.x * blockDim.x + threadIdx.x; m < n; m += gridDim.x * blockDim.x)
+    mprs[m] = fprs[par[m]];
 }
 
 // Copies the deleted tail [cs[tot], n) of double-buffered fields (fused
@@ -811,17 +826,15 @@ __device__ inline bool group_culled(real mx, real my, real mz, const Vec<real, D
 
 template <class real, int DIM>
 __global__ __launch_bounds__(64) void k_mirror_group(
-    const Vec<real, DIM>* __restrict__ fpos, const real* __restrict__ fprs,
-    const int* __restrict__ fid, const int* __restrict__ near_list,
+    const Vec<real, DIM>* __restrict__ fpos, const int* __restrict__ fid, const int* __restrict__ near_list,
     const int* __restrict__ near_scene, const int* __restrict__ near_count,
     const Cst<real>* __restrict__ tab, const Vec<real, DIM>* __restrict__ wpos,
     const Vec<real, DIM>* __restrict__ wnrm, const uint8_t* __restrict__ wcls,
-    const int* __restrict__ wcs, Vec<real, DIM>* __restrict__ mpos, real* __restrict__ mprs,
+    const int* __restrict__ wcs, Vec<real, DIM>* __restrict__ mpos, int* __restrict__ mpar,
     int* __restrict__ mid, int* __restrict__ mkey, int* __restrict__ mslot, int* __restrict__ cnt) {
   constexpr int P = 64 / kMG;
   __shared__ int first_reg[P][64], first_edge[P][64], sorted_reg[P][kNumMRR], sorted_edge[P][kNumMRR];
   __shared__ Vec<real, DIM> s_pos[P][kNumMRR];
-  __shared__ real s_prs[P][kNumMRR];
   __shared__ int s_id[P][kNumMRR], s_key[P][kNumMRR];
   const int lane = threadIdx.x, g = lane / kMG, gl = lane % kMG;
   const int nnear = *near_count;
@@ -877,14 +890,12 @@ __global__ __launch_bounds__(64) void k_mirror_group(
     const int ne = act ? min(__popcll(hite), kNumMRR) : 0;
     __syncthreads();
     const int base_id = fid[i] * kNumMRR;
-    const real prs_i = fprs[i];
     for (int k = 0; k < nreg; k++) {  // plane mirrors, kernel.cu:811-838
       const int F = sorted_reg[g][k];
       real mx, my, mz;
       if (!plane_mirror<real, DIM>(px, py, pz, wpos[F], wnrm[F], mx, my, mz)) continue;
       if (!group_culled<real, DIM>(mx, my, mz, wpos, wnrm, wcs, c, gl) && gl == 0) {
         s_pos[g][k] = mkvec<real, DIM>(mx, my, mz);
-        s_prs[g][k] = prs_i;
         s_id[g][k] = base_id + k;
         s_key[g][k] = mirror_key<real, DIM>(mx, my, mz, c);
       }
@@ -894,7 +905,6 @@ __global__ __launch_bounds__(64) void k_mirror_group(
       if (edge_mirror<real, DIM>(px, py, pz, sorted_edge[g], ne, wpos, wnrm, mx, my, mz) &&
           !group_culled<real, DIM>(mx, my, mz, wpos, wnrm, wcs, c, gl) && gl == 0) {
         s_pos[g][nreg] = mkvec<real, DIM>(mx, my, mz);
-        s_prs[g][nreg] = prs_i;
         s_id[g][nreg] = base_id + nreg;
         s_key[g][nreg] = mirror_key<real, DIM>(mx, my, mz, c);
       }
@@ -906,7 +916,7 @@ __global__ __launch_bounds__(64) void k_mirror_group(
       mkey[o] = k;
       if (k >= 0) {
         mpos[o] = s_pos[g][gl];
-        mprs[o] = s_prs[g][gl];
+        mpar[o] = i;  // the parent's place in the fluid set: its pressure is taken later
         mid[o] = s_id[g][gl];
         mslot[o] = atomicAdd(&cnt[k], 1);  // the mirror bucket build's count (as k_key_count)
       }
@@ -919,11 +929,10 @@ __global__ __launch_bounds__(64) void k_mirror_group(
 // per particle, emitting compacted raw entries.
 template <class real, int DIM>
 __global__ __launch_bounds__(kBlock) void k_mirror_gen(
-    const Vec<real, DIM>* __restrict__ fpos, const real* __restrict__ fprs,
-    const int* __restrict__ fid, const int* __restrict__ fcs, Scenes<real> sc,
-    const Vec<real, DIM>* __restrict__ wpos, const Vec<real, DIM>* __restrict__ wnrm,
+    const Vec<real, DIM>* __restrict__ fpos, const int* __restrict__ fid, const int* __restrict__ fcs,
+    Scenes<real> sc, const Vec<real, DIM>* __restrict__ wpos, const Vec<real, DIM>* __restrict__ wnrm,
     const int8_t* __restrict__ wedge, const int* __restrict__ wcs,
-    Vec<real, DIM>* __restrict__ mpos, real* __restrict__ mprs, int* __restrict__ mid,
+    Vec<real, DIM>* __restrict__ mpos, int* __restrict__ mpar, int* __restrict__ mid,
     int* __restrict__ mkey, int* __restrict__ mcount, int mcap) {
   int s, first, end;
   Cst<real> c;
@@ -965,13 +974,12 @@ __global__ __launch_bounds__(kBlock) void k_mirror_gen(
       }
     }
   if (!wexist) return;
-  const real prs_i = fprs[i];
   const int base_id = fid[i] * kNumMRR;
   auto emit = [&](real mx, real my, real mz, int slot_id) {
     const int slot = wave_alloc(mcount);
     if (slot < mcap) {
       mpos[slot] = mkvec<real, DIM>(mx, my, mz);
-      mprs[slot] = prs_i;
+      mpar[slot] = i;
       mid[slot] = slot_id;
       mkey[slot] = mirror_key<real, DIM>(mx, my, mz, c);
     }
