@@ -228,7 +228,6 @@ struct Impl : ImplBase {
   int *mid2 = nullptr, *mcs2 = nullptr, *mpar2 = nullptr;
   uint8_t* mnear2 = nullptr;
   int mswaps = 0;  // parity of swap_mirrors calls
-  bool capturing = false;  // a graph is being recorded: one stream only
   int *near_list = nullptr, *near_scene = nullptr;  // fluid particles with walls in their stencil
   int *nlist = nullptr, *ncount = nullptr;          // ordered fluid neighbours within rp
   mps::Chunk* chunks = nullptr;                     // per-particle hit records of VscTrm's scan
@@ -831,9 +830,10 @@ struct Impl : ImplBase {
       // bucket build (ChkCol changes velocities only) and the new pressures:
       // the mirror set is built on the side stream into the second set while
       // ChkCol + MkPrs read the current one; the pressures are filled in
-      // once MkPrs is done.  With graphs on, every step keeps one stream
-      // (a captured step must leave the mirror sets where it found them).
-      const bool overlap = nf > 0 && use_cls && !capturing && graph_steps == 0;
+      // once MkPrs is done.  Under graph capture the fork / join events
+      // bring the second stream into the graph; a captured run of steps (an
+      // even count) leaves the mirror sets where it found them.
+      const bool overlap = nf > 0 && use_cls;
       if (overlap) {
         (void)hipEventRecord(ev_fork, stream);
         (void)hipStreamWaitEvent(side, ev_fork, 0);
@@ -883,9 +883,7 @@ struct Impl : ImplBase {
     keys_ready = false;
     cnt_dirty = true;
     HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-    capturing = true;
     step_eager(graph_steps);
-    capturing = false;
     HIP_TRY(hipStreamEndCapture(stream, &graph));
     if (buffers() != graph_buffers) {
       drop_graph();
