@@ -789,9 +789,24 @@ __device__ inline void group_rows(const int* __restrict__ wcs, const Vec<real, D
   }
 }
 
+// Lower bound, in bucket widths, of the distance along one axis from
+// fractional cell coordinate t ((x - minc) / db, as cell_coord computes it)
+// to stencil row j, which holds the entries with t in [j - 1, j).  The slack
+// covers the rounding of t for the mirror and for the walls.
+template <class real>
+__device__ inline real slab_cells(real t, int j) {
+  const real d = max(t - real(j), real(j - 1) - t) - real(1.0 / 1024);
+  return d > real(0) ? d : real(0);
+}
+
 // Nearest wall within r of m for a group (first met wins ties): every lane
 // keeps the lexicographic (dist, index) minimum of its candidates and the
-// group reduction picks the overall one.  True when m is culled.
+// group reduction picks the overall one.  True when m is culled.  The
+// mirror's own stencil row goes first; the other rows are scanned only when
+// their slab can hold a wall as near as the best found there.  A lane's
+// prefilter d2 <= lim keeps every candidate whose rounded distance could tie
+// its best (lim is the best's d2 enlarged by 2^-20): candidates do not reach
+// a lane in index order, so a tie may still go to a smaller index.
 template <class real, int DIM>
 __device__ inline bool group_culled(real mx, real my, real mz, const Vec<real, DIM>* __restrict__ wpos,
                                     const Vec<real, DIM>* __restrict__ wnrm, const int* __restrict__ wcs,
@@ -800,25 +815,56 @@ __device__ inline bool group_culled(real mx, real my, real mz, const Vec<real, D
   real best = real(10) * c.r;
   real lim = c.r2_hi;
   int bj = -1;
-  group_rows<real, DIM>(wcs, wpos, nullptr, q, c, gl, true, [&](int j, const Vec<real, DIM>& w, int) {
+  auto visit = [&](int j, const Vec<real, DIM>& w, int) {
     const real d2 = norm2<real, DIM>(mx - w.x, my - w.y, mz - vz(w));
     if (!(d2 <= lim)) return;
     const real dist = sqrt(d2);
     if (dist <= c.r && (dist < best || (dist == best && j < bj))) {
       best = dist;
       bj = j;
-      lim = d2;
+      lim = d2 + d2 * real(1.0 / 1048576);
     }
-  });
-  for (int off = kMG / 2; off >= 1; off >>= 1) {
-    const real od = __shfl_xor(best, off);
-    const int oj = __shfl_xor(bj, off);
-    const bool take = oj >= 0 && (bj < 0 || od < best || (od == best && oj < bj));
-    if (take) {
-      best = od;
-      bj = oj;
+  };
+  auto reduce = [&]() {
+    for (int off = kMG / 2; off >= 1; off >>= 1) {
+      const real od = __shfl_xor(best, off);
+      const int oj = __shfl_xor(bj, off);
+      const bool take = oj >= 0 && (bj < 0 || od < best || (od == best && oj < bj));
+      if (take) {
+        best = od;
+        bj = oj;
+      }
     }
+  };
+  // one stencil row; a lane loads four of its candidates at a time
+  auto scan_row = [&](int jy, int jz) {
+    const int b = wcs[cell_key(q.ix - 1, jy, jz, c)], e = wcs[cell_key(q.ix + 1, jy, jz, c) + 1];
+    for (int j0 = b + gl; j0 < e; j0 += 4 * kMG) {
+      Vec<real, DIM> w[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int j = j0 + u * kMG;
+        w[u] = wpos[j < e ? j : b];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        if (j0 + u * kMG < e) visit(j0 + u * kMG, w[u], 0);
+    }
+  };
+  constexpr int NR = Rows<real, DIM>::NR, kCenter = NR / 2;
+  scan_row(q.iy, DIM == 3 ? q.iz : 0);
+  reduce();
+  // the rows whose slab lies within best of m (group-uniform, as best is)
+  const real ty = (my - c.minc[1]) * c.dbinv;
+  const real tz = DIM == 3 ? (mz - c.minc[2]) * c.dbinv : real(0);
+  const real reach = best * c.dbinv;
+  for (int r = 0; r < NR; r++) {
+    if (r == kCenter) continue;
+    const int jz = DIM == 3 ? q.iz - 1 + r / 3 : 0, jy = q.iy - 1 + r % 3;
+    const real by = slab_cells(ty, jy), bz = DIM == 3 ? slab_cells(tz, jz) : real(0);
+    if (by * by + bz * bz <= reach * reach) scan_row(jy, jz);
   }
+  reduce();
   if (bj < 0) return false;
   const Vec<real, DIM> w = wpos[bj], n = wnrm[bj];
   return dot3<real, DIM>(mx - w.x, my - w.y, mz - vz(w), n.x, n.y, vz(n)) >= real(0);
