@@ -763,7 +763,12 @@ struct Impl : ImplBase {
 
   // ChkCol + UpChkCol and MkPrs in one launch (the step's fused path):
   // new velocities into fvel[1], swapped in; pressure and number density.
-  void run_colprs() {
+  void run_colprs(bool scan = false) {
+    if (scan)  // builds the neighbour list too
+      TLAUNCH("col_prs", (mps::k_colprs_list<real, DIM, true>), dim3(sb_grid), dim3(block), 0, stream, fpos[0],
+              fvel[0], (const int*)fcs, nf, nlist, ncount, wpos, wvel, wcs, mpos, mcs, (const uint8_t*)wnear,
+              (const uint8_t*)mnear, sc(), fvel[1], fprs[0], fnd[0], tail(false));
+    else
     TLAUNCH("col_prs", (mps::k_colprs_list<real, DIM>), dim3(sb_grid), dim3(block), 0, stream, fpos[0],
             fvel[0], (const int*)fcs, nf, nlist, ncount, wpos, wvel, wcs, mpos, mcs, (const uint8_t*)wnear,
             (const uint8_t*)mnear, sc(), fvel[1], fprs[0], fnd[0], tail(false));
@@ -844,10 +849,7 @@ struct Impl : ImplBase {
         swap_mirrors();
         (void)hipEventRecord(ev_join, side);
       }
-      if (nf > 0) {
-        run_nlist();
-        run_colprs();
-      }
+      if (nf > 0) run_colprs(true);
       if (overlap) {
         (void)hipStreamWaitEvent(stream, ev_join, 0);
         swap_mirrors();

@@ -122,21 +122,27 @@ constexpr int stage_cap() {
   return kStageBytes / int(sizeof(V)) - 8;  // 8 spare entries for the scan's overrun
 }
 
-template <class V, class Test, class Emit>
+// emit(base, mask) or, with SRC, emit(base, mask, src) where src[j] is
+// candidate j (the LDS copy when staged, else arr itself).
+template <bool SRC = false, class V, class Test, class Emit>
 __device__ inline void scan_row_staged(V* stage, int b, int e, bool act, const V* __restrict__ arr,
                                        Test&& test, Emit&& emit) {
   const int B = wave_min(act ? b : 0x7fffffff), E = wave_max(act ? e : -1);
   if (E <= B) return;
+  auto run = [&](const V* src) {
+    if constexpr (SRC)
+      scan_chunks(b, e, [&](int j) { return test(src[j]); },
+                  [&](int base, unsigned long long m) { emit(base, m, src); });
+    else
+      scan_chunks(b, e, [&](int j) { return test(src[j]); }, emit);
+  };
   if (E - B <= stage_cap<V>()) {
     for (int t = __lane_id(); t < E - B; t += 64) stage[t] = arr[B + t];
     __builtin_amdgcn_wave_barrier();
-    if (act) {
-      const V* s = stage - B;
-      scan_chunks(b, e, [&](int j) { return test(s[j]); }, emit);
-    }
+    if (act) run(stage - B);
     __builtin_amdgcn_wave_barrier();
   } else if (act) {
-    scan_chunks(b, e, [&](int j) { return test(arr[j]); }, emit);
+    run(arr);
   }
 }
 

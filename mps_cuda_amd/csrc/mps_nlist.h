@@ -136,10 +136,12 @@ __device__ inline void for_mirrors_near(Vec<real, DIM>* stage, bool act, real px
   for (int jz = zlo; jz <= zhi; jz++)
     for (int jy = q.iy - 1; jy <= q.iy + 1; jy++) {
       const int b = mcs[cell_key(q.ix - 1, jy, jz, c)], e = mcs[cell_key(q.ix + 1, jy, jz, c) + 1];
-      scan_row_staged(
+      scan_row_staged<true>(
           stage, b, e, act, mpos,
           [&](const Vec<real, DIM>& pj) { return near2(pj.x - px, pj.y - py, vz(pj) - pz) <= lim; },
-          [&](int base, unsigned long long m) { for_bits(base, m, [&](int j) { f(j, mpos[j]); }); });
+          [&](int base, unsigned long long m, const Vec<real, DIM>* src) {
+            for_bits(base, m, [&](int j) { f(j, src[j]); });
+          });
     }
 }
 
@@ -154,11 +156,14 @@ __device__ inline void for_mirrors_near(Vec<real, DIM>* stage, bool act, real px
 // stencil applies fluid contacts during the walk; one with walls applies its
 // wall contacts after the walk when it had no fluid contact, and otherwise
 // replays the reference's bucket-by-bucket order.
-template <class real, int DIM>
+// SCAN: the kernel builds the list itself (k_nlist_build's staged scan, the
+// same entries in the same order) and treats each fluid candidate as the
+// scan meets it, in list order; the list then serves PrsGrdTrm.
+template <class real, int DIM, bool SCAN = false>
 __global__ __launch_bounds__(kBlock) void k_colprs_list(
     const Vec<real, DIM>* __restrict__ pos, const Vec<real, DIM>* __restrict__ vel,
-    const int* __restrict__ fcs, int n_alloc, const int* __restrict__ list,
-    const int* __restrict__ count, const Vec<real, DIM>* __restrict__ wpos,
+    const int* __restrict__ fcs, int n_alloc, int* __restrict__ list,
+    int* __restrict__ count, const Vec<real, DIM>* __restrict__ wpos,
     const Vec<real, DIM>* __restrict__ wvel, const int* __restrict__ wcs,
     const Vec<real, DIM>* __restrict__ mpos, const int* __restrict__ mcs,
     const uint8_t* __restrict__ wnear, const uint8_t* __restrict__ mnear, Scenes<real> sc,
@@ -202,23 +207,46 @@ __global__ __launch_bounds__(kBlock) void k_colprs_list(
   const Cell<real, DIM> q = cell_of<real, DIM>(p, c);
   const int hk = cell_key(q.ix, q.iy, q.iz, c);
   const bool walls = act && wnear[hk];
-  const int n = count[i];
   int fluid_contacts = 0;
-  if (act)
-  for_near<real, DIM>(i, px, py, pz, (size_t)n_alloc, list, n, pos, fcs, q, c,
-                      [&](int j, const Vec<real, DIM>& pj, real) {
-                        const real qx = pj.x - px, qy = pj.y - py, qz = vz(pj) - pz;
-                        const real d2 = norm2<real, DIM>(qx, qy, qz);
-                        if (d2 <= lim) {
-                          const real dist = sqrt(d2);
-                          if (dist < rp) ni += (rp / dist) - (dist / rp);
-                        }
-                        if (d2 < rlim2) {
-                          if (!walls) apply(qx, qy, qz, d2, vel[j]);
-                          fluid_contacts++;
-                        }
-                      });
+  auto fluid = [&](int j, const Vec<real, DIM>& pj) {
+    const real qx = pj.x - px, qy = pj.y - py, qz = vz(pj) - pz;
+    const real d2 = norm2<real, DIM>(qx, qy, qz);
+    if (d2 <= lim) {
+      const real dist = sqrt(d2);
+      if (dist < rp) ni += (rp / dist) - (dist / rp);
+    }
+    if (d2 < rlim2) {
+      if (!walls) apply(qx, qy, qz, d2, vel[j]);
+      fluid_contacts++;
+    }
+  };
   const int zlo = DIM == 3 ? q.iz - 1 : 0, zhi = DIM == 3 ? q.iz + 1 : 0;
+  int n;
+  if constexpr (SCAN) {
+    n = 0;
+    for (int jz = zlo; jz <= zhi; jz++)
+      for (int jy = q.iy - 1; jy <= q.iy + 1; jy++) {
+        const int b = fcs[cell_key(q.ix - 1, jy, jz, c)], e = fcs[cell_key(q.ix + 1, jy, jz, c) + 1];
+        scan_row_staged<true>(
+            stage, b, e, act, pos,
+            [&](const V& pj) { return near2(pj.x - px, pj.y - py, vz(pj) - pz) <= lim; },
+            [&](int base, unsigned long long m, const V* src) {
+              if (unsigned(i - base) < 64u) m &= ~(1ull << (i - base));  // j != i
+              for_bits(base, m, [&](int j) {
+                if (n < kNList) list[(size_t)n * n_alloc + i] = j;
+                n++;
+                fluid(j, src[j]);
+              });
+            });
+      }
+    if (n > kNList) n = -1;
+    if (act) count[i] = n;
+  } else {
+    n = count[i];
+    if (act)
+      for_near<real, DIM>(i, px, py, pz, (size_t)n_alloc, list, n, pos, fcs, q, c,
+                          [&](int j, const Vec<real, DIM>& pj, real) { fluid(j, pj); });
+  }
   const bool wscan = walls && fluid_contacts == 0;
   if (__ballot(wscan)) {
     const real wl = c.rlim_wall2;
