@@ -700,9 +700,18 @@ struct Impl : ImplBase {
     mswaps ^= 1;
   }
 
+  hipError_t order_err = hipSuccess;  // first failed fork / join of the two streams
+  void order(hipError_t e) {
+    if (e != hipSuccess && order_err == hipSuccess) order_err = e;
+  }
   int check_launch() {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return set_err(MPS_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+    if (order_err != hipSuccess) {
+      e = order_err;
+      order_err = hipSuccess;
+      return set_err(MPS_E_HIP, std::string("stream fork / join: ") + hipGetErrorString(e));
+    }
     return MPS_OK;
   }
 
@@ -840,18 +849,18 @@ struct Impl : ImplBase {
       // even count) leaves the mirror sets where it found them.
       const bool overlap = nf > 0 && use_cls;
       if (overlap) {
-        (void)hipEventRecord(ev_fork, stream);
-        (void)hipStreamWaitEvent(side, ev_fork, 0);
+        order(hipEventRecord(ev_fork, stream));
+        order(hipStreamWaitEvent(side, ev_fork, 0));
         swap_mirrors();
         std::swap(stream, side);
         mirror(true, false);
         std::swap(stream, side);
         swap_mirrors();
-        (void)hipEventRecord(ev_join, side);
+        order(hipEventRecord(ev_join, side));
       }
       if (nf > 0) run_colprs(true);
       if (overlap) {
-        (void)hipStreamWaitEvent(stream, ev_join, 0);
+        order(hipStreamWaitEvent(stream, ev_join, 0));
         swap_mirrors();
         hipLaunchKernelGGL((mps::k_mirror_prs<real>), dim3(nblk(mcap, 2048)), B, 0, stream,
                            (const int*)(mcs + tot), (const int*)mpar, (const real*)fprs[0], mprs);

@@ -122,27 +122,29 @@ constexpr int stage_cap() {
   return kStageBytes / int(sizeof(V)) - 8;  // 8 spare entries for the scan's overrun
 }
 
-// emit(base, mask) or, with SRC, emit(base, mask, src) where src[j] is
-// candidate j (the LDS copy when staged, else arr itself).
+// emit(base, mask) or, with SRC, emit(base, mask, src, off) where candidate
+// j is src[j - off] (the LDS copy of the row when staged, else arr itself).
+// Candidate addresses are always formed as a pointer plus a non-negative
+// index: a pointer such as stage - B would lie outside the LDS window.
 template <bool SRC = false, class V, class Test, class Emit>
 __device__ inline void scan_row_staged(V* stage, int b, int e, bool act, const V* __restrict__ arr,
                                        Test&& test, Emit&& emit) {
   const int B = wave_min(act ? b : 0x7fffffff), E = wave_max(act ? e : -1);
   if (E <= B) return;
-  auto run = [&](const V* src) {
+  auto run = [&](const V* src, int off) {
     if constexpr (SRC)
-      scan_chunks(b, e, [&](int j) { return test(src[j]); },
-                  [&](int base, unsigned long long m) { emit(base, m, src); });
+      scan_chunks(b, e, [&](int j) { return test(src[j - off]); },
+                  [&](int base, unsigned long long m) { emit(base, m, src, off); });
     else
-      scan_chunks(b, e, [&](int j) { return test(src[j]); }, emit);
+      scan_chunks(b, e, [&](int j) { return test(src[j - off]); }, emit);
   };
   if (E - B <= stage_cap<V>()) {
     for (int t = __lane_id(); t < E - B; t += 64) stage[t] = arr[B + t];
     __builtin_amdgcn_wave_barrier();
-    if (act) run(stage - B);
+    if (act) run(stage, B);
     __builtin_amdgcn_wave_barrier();
   } else if (act) {
-    run(arr);
+    run(arr, 0);
   }
 }
 
@@ -1185,16 +1187,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
     chunks[(size_t)nch * n_alloc + i] = Chunk{base, 0, m};
     nch++;
   };
+  auto fluid_emit = [&](int base, unsigned long long m) {
+    if (unsigned(i - base) < 64u) m &= ~(1ull << (i - base));  // j != i
+    if (m) emit(base, m);
+  };
   for (int jz = zlo; jz <= zhi; jz++)
     for (int jy = q.iy - 1; jy <= q.iy + 1; jy++) {
       const int b = fcs[cell_key(q.ix - 1, jy, jz, c)], e = fcs[cell_key(q.ix + 1, jy, jz, c) + 1];
       scan_row_staged(
           stage, b, e, act, pos,
-          [&](const V& pj) { return near2(pj.x - px, pj.y - py, vz(pj) - pz) <= r2_hi; },
-          [&](int base, unsigned long long m) {
-            if (unsigned(i - base) < 64u) m &= ~(1ull << (i - base));  // j != i
-            if (m) emit(base, m);
-          });
+          [&](const V& pj) { return near2(pj.x - px, pj.y - py, vz(pj) - pz) <= r2_hi; }, fluid_emit);
     }
   const bool mact = act && mnear[cell_key(q.ix, q.iy, q.iz, c)];
   if (__ballot(mact))

@@ -36,19 +36,19 @@ __global__ __launch_bounds__(kBlock) void k_nlist_build(const Vec<real, DIM>* __
   const Cell<real, DIM> q = cell_of<real, DIM>(p, c);
   const int zlo = DIM == 3 ? q.iz - 1 : 0, zhi = DIM == 3 ? q.iz + 1 : 0;
   int n = 0;
+  auto emit = [&](int base, unsigned long long m) {
+    if (unsigned(i - base) < 64u) m &= ~(1ull << (i - base));  // j != i
+    for_bits(base, m, [&](int j) {
+      if (n < kNList) list[(size_t)n * n_alloc + i] = j;
+      n++;
+    });
+  };
   for (int jz = zlo; jz <= zhi; jz++)
     for (int jy = q.iy - 1; jy <= q.iy + 1; jy++) {
       const int b = fcs[cell_key(q.ix - 1, jy, jz, c)], e = fcs[cell_key(q.ix + 1, jy, jz, c) + 1];
       scan_row_staged(
           stage, b, e, act, pos,
-          [&](const V& pj) { return near2(pj.x - px, pj.y - py, vz(pj) - pz) <= lim; },
-          [&](int base, unsigned long long m) {
-            if (unsigned(i - base) < 64u) m &= ~(1ull << (i - base));  // j != i
-            for_bits(base, m, [&](int j) {
-              if (n < kNList) list[(size_t)n * n_alloc + i] = j;
-              n++;
-            });
-          });
+          [&](const V& pj) { return near2(pj.x - px, pj.y - py, vz(pj) - pz) <= lim; }, emit);
     }
   if (act) count[i] = n <= kNList ? n : -1;
 }
@@ -139,8 +139,8 @@ __device__ inline void for_mirrors_near(Vec<real, DIM>* stage, bool act, real px
       scan_row_staged<true>(
           stage, b, e, act, mpos,
           [&](const Vec<real, DIM>& pj) { return near2(pj.x - px, pj.y - py, vz(pj) - pz) <= lim; },
-          [&](int base, unsigned long long m, const Vec<real, DIM>* src) {
-            for_bits(base, m, [&](int j) { f(j, src[j]); });
+          [&](int base, unsigned long long m, const Vec<real, DIM>* src, int off) {
+            for_bits(base, m, [&](int j) { f(j, src[j - off]); });
           });
     }
 }
@@ -224,20 +224,20 @@ __global__ __launch_bounds__(kBlock) void k_colprs_list(
   int n;
   if constexpr (SCAN) {
     n = 0;
+    auto emit = [&](int base, unsigned long long m, const V* src, int off) {
+      if (unsigned(i - base) < 64u) m &= ~(1ull << (i - base));  // j != i
+      for_bits(base, m, [&](int j) {
+        if (n < kNList) list[(size_t)n * n_alloc + i] = j;
+        n++;
+        fluid(j, src[j - off]);
+      });
+    };
     for (int jz = zlo; jz <= zhi; jz++)
       for (int jy = q.iy - 1; jy <= q.iy + 1; jy++) {
         const int b = fcs[cell_key(q.ix - 1, jy, jz, c)], e = fcs[cell_key(q.ix + 1, jy, jz, c) + 1];
         scan_row_staged<true>(
             stage, b, e, act, pos,
-            [&](const V& pj) { return near2(pj.x - px, pj.y - py, vz(pj) - pz) <= lim; },
-            [&](int base, unsigned long long m, const V* src) {
-              if (unsigned(i - base) < 64u) m &= ~(1ull << (i - base));  // j != i
-              for_bits(base, m, [&](int j) {
-                if (n < kNList) list[(size_t)n * n_alloc + i] = j;
-                n++;
-                fluid(j, src[j]);
-              });
-            });
+            [&](const V& pj) { return near2(pj.x - px, pj.y - py, vz(pj) - pz) <= lim; }, emit);
       }
     if (n > kNList) n = -1;
     if (act) count[i] = n;
