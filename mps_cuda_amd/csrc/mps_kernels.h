@@ -892,7 +892,12 @@ __global__ __launch_bounds__(64) void k_mirror_group(
   __shared__ int s_id[P][kNumMRR], s_key[P][kNumMRR];
   const int lane = threadIdx.x, g = lane / kMG, gl = lane % kMG;
   const int nnear = *near_count;
-  for (int t0 = blockIdx.x * P; t0 < nnear; t0 += gridDim.x * P) {
+  // XCD-aware order over the workgroups that have work: each XCD takes an
+  // equal, contiguous share of the (bucket-sorted) list, so neighbouring
+  // particles, which read the same walls, share an L2
+  const int G = min(int(gridDim.x), (nnear + P - 1) / P);
+  if (int(blockIdx.x) >= G) return;  // whole workgroup (one wavefront)
+  for (int t0 = xcd_tile(blockIdx.x, G) * P; t0 < nnear; t0 += G * P) {
     const int t = t0 + g;
     const bool act = t < nnear;
     const int tt = act ? t : t0;
