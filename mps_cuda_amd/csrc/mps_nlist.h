@@ -16,6 +16,10 @@
 namespace mps {
 
 constexpr int kNList = 64;
+// Bit 31 of a list entry: the candidate lies within d2 < rp2 of the particle
+// (norm2(pj - p) < rp2, PrsGrdTrm's test of its minimum-pressure pass);
+// readers mask it off (kListIdx).
+constexpr int kInRp = int(0x80000000u), kListIdx = 0x7fffffff;
 
 template <class real, int DIM>
 __global__ __launch_bounds__(kBlock) void k_nlist_build(const Vec<real, DIM>* __restrict__ pos,
@@ -36,17 +40,19 @@ __global__ __launch_bounds__(kBlock) void k_nlist_build(const Vec<real, DIM>* __
   const Cell<real, DIM> q = cell_of<real, DIM>(p, c);
   const int zlo = DIM == 3 ? q.iz - 1 : 0, zhi = DIM == 3 ? q.iz + 1 : 0;
   int n = 0;
-  auto emit = [&](int base, unsigned long long m) {
+  auto emit = [&](int base, unsigned long long m, const V* src, int off) {
     if (unsigned(i - base) < 64u) m &= ~(1ull << (i - base));  // j != i
     for_bits(base, m, [&](int j) {
-      if (n < kNList) list[(size_t)n * n_alloc + i] = j;
+      const V pj = src[j - off];
+      const bool in_rp = norm2<real, DIM>(pj.x - px, pj.y - py, vz(pj) - pz) < c.rp2;
+      if (n < kNList) list[(size_t)n * n_alloc + i] = j | (in_rp ? kInRp : 0);
       n++;
     });
   };
   for (int jz = zlo; jz <= zhi; jz++)
     for (int jy = q.iy - 1; jy <= q.iy + 1; jy++) {
       const int b = fcs[cell_key(q.ix - 1, jy, jz, c)], e = fcs[cell_key(q.ix + 1, jy, jz, c) + 1];
-      scan_row_staged(
+      scan_row_staged<true>(
           stage, b, e, act, pos,
           [&](const V& pj) { return near2(pj.x - px, pj.y - py, vz(pj) - pz) <= lim; }, emit);
     }
@@ -61,7 +67,7 @@ __device__ inline void for_fluid_near(int i, int n_alloc, const int* __restrict_
                                       const Cell<real, DIM>& q, const Cst<real>& c, F&& f) {
   const int n = count[i];
   if (n >= 0) {
-    for (int k = 0; k < n; k++) f(list[(size_t)k * n_alloc + i]);
+    for (int k = 0; k < n; k++) f(list[(size_t)k * n_alloc + i] & kListIdx);
     return;
   }
   const int zlo = DIM == 3 ? q.iz - 1 : 0, zhi = DIM == 3 ? q.iz + 1 : 0;
@@ -93,7 +99,7 @@ __device__ inline void for_near(int i, real px, real py, real pz, size_t n_alloc
       Vec<real, DIM> pj[8];
       real a[8];
 #pragma unroll
-      for (int u = 0; u < 8; u++) j[u] = list[(size_t)min(k + u, n - 1) * n_alloc + i];
+      for (int u = 0; u < 8; u++) j[u] = list[(size_t)min(k + u, n - 1) * n_alloc + i] & kListIdx;
 #pragma unroll
       for (int u = 0; u < 8; u++) {
         pj[u] = pos[j[u]];
@@ -227,9 +233,11 @@ __global__ __launch_bounds__(kBlock) void k_colprs_list(
     auto emit = [&](int base, unsigned long long m, const V* src, int off) {
       if (unsigned(i - base) < 64u) m &= ~(1ull << (i - base));  // j != i
       for_bits(base, m, [&](int j) {
-        if (n < kNList) list[(size_t)n * n_alloc + i] = j;
+        const V pj = src[j - off];
+        const bool in_rp = norm2<real, DIM>(pj.x - px, pj.y - py, vz(pj) - pz) < c.rp2;
+        if (n < kNList) list[(size_t)n * n_alloc + i] = j | (in_rp ? kInRp : 0);
         n++;
-        fluid(j, src[j - off]);
+        fluid(j, pj);
       });
     };
     for (int jz = zlo; jz <= zhi; jz++)
@@ -271,7 +279,7 @@ __global__ __launch_bounds__(kBlock) void k_colprs_list(
             const int fe = fcs[ck + 1];
             if (n >= 0) {
               for (; k < n; k++) {
-                const int j = list[(size_t)k * n_alloc + i];
+                const int j = list[(size_t)k * n_alloc + i] & kListIdx;
                 if (j >= fe) break;
                 contact(pos[j], [&] { return vel[j]; }, rlim2);
               }
@@ -354,7 +362,7 @@ __global__ __launch_bounds__(kBlock) void k_col_list(
   if (n >= 0 && !wnear[hk]) {
     // no wall in the stencil: the list alone is the visiting order
     for (; k < n; k++) {
-      const int j = list[(size_t)k * n_alloc + i];
+      const int j = list[(size_t)k * n_alloc + i] & kListIdx;
       contact(pos[j], [&] { return vel[j]; }, c.rlim2);
     }
   } else
@@ -365,7 +373,7 @@ __global__ __launch_bounds__(kBlock) void k_col_list(
         const int fe = fcs[ck + 1];
         if (n >= 0) {
           for (; k < n; k++) {
-            const int j = list[(size_t)k * n_alloc + i];
+            const int j = list[(size_t)k * n_alloc + i] & kListIdx;
             if (j >= fe) break;
             contact(pos[j], [&] { return vel[j]; }, c.rlim2);
           }
@@ -478,14 +486,30 @@ __global__ __launch_bounds__(kBlock) void k_grad_list(
   real ax = 0, ay = 0, az = 0;
   const Cell<real, DIM> q = cell_of<real, DIM>(p, c);
   const int n = count[i];
-  // P_min over fluid neighbours with d2 < rp2 (kernel.cu:1686-1700)
-  if (act)
-  for_near<real, DIM>(
-      i, px, py, pz, (size_t)n_alloc, list, n, pos, fcs, q, c,
-      [&](int, const Vec<real, DIM>& pj, real pr) {
-        if (norm2<real, DIM>(pj.x - px, pj.y - py, vz(pj) - pz) < rp2 && pr < Prs_min) Prs_min = pr;
-      },
-      prs);
+  // P_min over fluid neighbours with d2 < rp2 (kernel.cu:1686-1700): from a
+  // complete list, the entries flagged kInRp (the same test, made when the
+  // list was built from these positions); a minimum does not depend on the
+  // order it is taken in.
+  if (act && n >= 0) {
+    for (int k = 0; k < n; k += 8) {
+      int e[8];
+      real pr[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) e[u] = list[(size_t)min(k + u, n - 1) * n_alloc + i];
+#pragma unroll
+      for (int u = 0; u < 8; u++) pr[u] = prs[e[u] & kListIdx];
+#pragma unroll
+      for (int u = 0; u < 8; u++)
+        if (k + u < n && e[u] < 0 && pr[u] < Prs_min) Prs_min = pr[u];
+    }
+  } else if (act) {
+    for_near<real, DIM>(
+        i, px, py, pz, (size_t)n_alloc, list, n, pos, fcs, q, c,
+        [&](int, const Vec<real, DIM>& pj, real pr) {
+          if (norm2<real, DIM>(pj.x - px, pj.y - py, vz(pj) - pz) < rp2 && pr < Prs_min) Prs_min = pr;
+        },
+        prs);
+  }
   // moment matrix (TENSOR): row-major, the 2x2 block in 2-D
   real m00 = 0, m01 = 0, m02 = 0, m10 = 0, m11 = 0, m12 = 0, m20 = 0, m21 = 0, m22 = 0;
   auto add = [&](const Vec<real, DIM>& pj, real pr) {
