@@ -436,6 +436,7 @@ int main(int argc, char** argv) {
   std::printf("SetPara: n0=%.9g n0_grad=%.9g lambda=%.9g nB=%dx%dx%d dt=%.9g c=%.9g umax=%.9g Pmax=%.9g\n",
               k.n0, k.n0_grad, k.lmd, k.nb[0], k.nb[1], k.nb[2], k.dt, k.snd, k.umax, k.pmax);
   std::printf("fluid %d, walls %d\n", k.n_fluid, k.n_wall);
+  std::printf("pressure gradient: %s\n", p.grad_model == MPS_GRAD_TENSOR ? "tensor" : "standard");
   if (o.dry) return 0;
 
   mps_handle* h = nullptr;

@@ -84,6 +84,14 @@ def test_reader_drops_particles_outside_the_domain(tmp_path):
     assert f"RdDat: {n} particles (2 outside the domain dropped)" in out
 
 
+def test_driver_gradient_model_option():
+    assert "pressure gradient: standard" in run("--dambreak", "cfg0", "--dry-run")
+    assert "pressure gradient: tensor" in run("--dambreak", "cfg0", "--dry-run", "--grad", "tensor")
+    res = subprocess.run([DRIVER, "--dambreak", "cfg0", "--dry-run", "--grad", "nope"], capture_output=True,
+                         text=True)
+    assert res.returncode == 2
+
+
 def test_driver_rejects_bad_arguments():
     res = subprocess.run([DRIVER, "--dambreak", "nope", "--dry-run"], capture_output=True, text=True)
     assert res.returncode != 0 and "unknown" in res.stderr

@@ -137,3 +137,28 @@ def test_launch_shape_does_not_change_results(block, wave_grid, kind):
     h.step(10)
     assert_states_equal(h.state(), ref.state(), f"block {block}", STEP_FIELDS)
 
+
+def test_caller_stream_equals_own_stream():
+    """mps_set_stream: the step's work (including the mirror build it forks
+    onto its second stream) follows the caller's stream, bit for bit.  The
+    stream comes from the HIP runtime the library itself uses."""
+    import ctypes as C
+
+    sc, p = configs.small3d("fp32", jitter=0.01, seed=9)
+    print("[case] caller stream", flush=True)
+    own = ffi.Handle(p, sc)
+    caller = ffi.Handle(p, sc)
+    hip = C.CDLL("libamdhip64.so")
+    stream = C.c_void_p()
+    assert hip.hipStreamCreate(C.byref(stream)) == 0
+    caller.set_stream(stream.value)
+    for chunk in (1, 6, 13):
+        own.step(chunk)
+        caller.step(chunk)
+        assert_states_equal(caller.state(), own.state(), f"caller stream after {chunk}", STEP_FIELDS)
+    caller.set_stream(0)
+    own.step(4)
+    caller.step(4)
+    assert_states_equal(caller.state(), own.state(), "back on the own stream", STEP_FIELDS)
+    hip.hipStreamDestroy.argtypes = [C.c_void_p]
+    assert hip.hipStreamDestroy(stream) == 0

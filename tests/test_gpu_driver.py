@@ -57,6 +57,14 @@ def test_driver_step_mode_outputs(tmp_path):
     assert np.array_equal(walls[:, 5].astype(np.int8), ffi.compute_edge(p, sc)[sc.typ == ffi.WLL])
 
 
+def test_driver_tensor_gradient(tmp_path):
+    drive("--dambreak", "small2d", "--steps", "20", "--every", "20", "--grad", "tensor", "--out", str(tmp_path))
+    sc, p = configs.small2d("fp64", grad_model=ffi.GRAD_TENSOR)
+    h = ffi.Handle(p, sc)
+    h.step(20)
+    check_output(tmp_path / "output00001.csv", h.state(), 20.0)
+
+
 def test_driver_file_mode_clock(tmp_path):
     ref = tmp_path / "ref"
     out = tmp_path / "out"
