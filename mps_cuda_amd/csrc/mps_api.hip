@@ -642,10 +642,10 @@ struct Impl : ImplBase {
   }
 
   // ResetMRR + GenMRR_nonslip + MkBkt_MRR (kernel.cu:1906-1908).  `fresh`:
-  // no position has changed since the last bucket(), whose gather left the
-  // near-wall list in near_list / mcount; otherwise it is built here.
-  // Without normal classes mcount is zero here (the general generator counts
-  // from zero; the mirror build's last kernel clears it).
+  // no position has changed since the last bucket(), whose scan and gather
+  // left the near-wall list in near_list / mcount; otherwise it is built
+  // here.  Without normal classes mcount is zero here (the general
+  // generator counts from zero; the mirror build's last kernel clears it).
   // with_prs: the mirror pressures are taken now; otherwise k_mirror_prs
   // fills them later through mpar.
   int mirror(bool fresh, bool with_prs = true) {

@@ -694,8 +694,10 @@ __device__ inline int mirror_key(real mx, real my, real mz, const Cst<real>& c) 
 // ---------------------------------------------------------------------------
 // Group-parallel mirror generation (at most 64 wall-normal classes).
 //
-// k_mirror_near lists the live fluid particles whose stencil holds any wall.
-// k_mirror_group then gives each listed particle a group of kMG lanes
+// The near-wall list holds the live fluid particles whose stencil holds any
+// wall: the fluid bucket build's gather writes it (k_gather_fluid), or
+// k_mirror_near when positions changed since.  k_mirror_group then gives
+// each listed particle a group of kMG lanes
 // (64 / kMG particles per wavefront):
 //  * the lanes split the wall candidates of the stencil; in the reference's
 //    scan order (z, y, x cells, then id) the candidates appear in increasing
@@ -709,7 +711,9 @@ __device__ inline int mirror_key(real mx, real my, real mz, const Cst<real>& c) 
 //    keep the lexicographic (dist, index) minimum of their share, and a group
 //    reduction picks the overall one, which is the reference's first-met
 //    minimum.
-//  * slot k of listed particle t goes to raw entry 10*t + k (key -1 if empty).
+//  * slot k of listed particle t goes to raw entry 10*t + k (key -1 if empty),
+//    with the parent's place (its pressure is read later) and the counting
+//    slot of its bucket key.
 
 constexpr int kInf = 0x7fffffff;
 
