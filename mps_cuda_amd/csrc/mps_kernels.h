@@ -774,15 +774,19 @@ constexpr int kMG = 16;
 static_assert(kMG >= kNumMRR && (kMG & (kMG - 1)) == 0 && 64 % kMG == 0, "group width");
 
 // visit(j, wpos[j], wcls[j]) for every wall candidate of the stencil rows of
-// q; lane gl of the group takes candidates b + gl, b + gl + kMG, ... of every
-// row, one per row per round, all rows' loads of a round issued together.
+// q that are set in `rows`; lane gl of the group takes candidates b + gl,
+// b + gl + kMG, ... of every row, one per row per round, all rows' loads of a
+// round issued together.
 template <class real, int DIM, class Visit>
 __device__ inline void group_rows(const int* __restrict__ wcs, const Vec<real, DIM>* __restrict__ wpos,
                                   const uint8_t* __restrict__ wcls, const Cell<real, DIM>& q,
-                                  const Cst<real>& c, int gl, bool act, Visit&& visit) {
+                                  const Cst<real>& c, int gl, bool act, Visit&& visit, unsigned rows = ~0u) {
   if (!act) return;
-  const Rows<real, DIM> R(wcs, q, c);
+  Rows<real, DIM> R(wcs, q, c);
   constexpr int NR = Rows<real, DIM>::NR;
+#pragma unroll
+  for (int r = 0; r < NR; r++)
+    if (!((rows >> r) & 1u)) R.e[r] = R.b[r];
   for (int off = 0;; off += kMG) {
     Vec<real, DIM> w[NR];
     int cl[NR];
@@ -917,12 +921,26 @@ __global__ __launch_bounds__(64) void k_mirror_group(
     if (gl < kNumMRR) s_key[g][gl] = -1;
     __syncthreads();
     const real r2_2 = real(1) * c.r2;
-    group_rows<real, DIM>(wcs, wpos, wcls, q, c, gl, act, [&](int j, const Vec<real, DIM>& w, int cl) {
-      if (norm2<real, DIM>(px - w.x, py - w.y, pz - vz(w)) <= r2_2) {
-        atomicMin(&first_reg[g][cl & 63], j);
-        if (cl & 0x80) atomicMin(&first_edge[g][cl & 63], j);
+    // rows whose slab lies beyond r (corner rows only) hold no wall within r
+    unsigned rows = 0;
+    {
+      const real ty = (py - c.minc[1]) * c.dbinv, tz = DIM == 3 ? (pz - c.minc[2]) * c.dbinv : real(0);
+      const real reach = c.r * c.dbinv;
+      for (int r = 0; r < Rows<real, DIM>::NR; r++) {
+        const int jz = DIM == 3 ? q.iz - 1 + r / 3 : 0, jy = q.iy - 1 + r % 3;
+        const real by = slab_cells(ty, jy), bz = DIM == 3 ? slab_cells(tz, jz) : real(0);
+        if (by * by + bz * bz <= reach * reach) rows |= 1u << r;
       }
-    });
+    }
+    group_rows<real, DIM>(
+        wcs, wpos, wcls, q, c, gl, act,
+        [&](int j, const Vec<real, DIM>& w, int cl) {
+          if (norm2<real, DIM>(px - w.x, py - w.y, pz - vz(w)) <= r2_2) {
+            atomicMin(&first_reg[g][cl & 63], j);
+            if (cl & 0x80) atomicMin(&first_edge[g][cl & 63], j);
+          }
+        },
+        rows);
     __syncthreads();
     // registry order: rank of each hit class's first index among the group's hit classes
     unsigned long long hit = 0, hite = 0;
