@@ -236,6 +236,7 @@ struct Impl : ImplBase {
   int *key = nullptr, *slot = nullptr, *tmp = nullptr, *tmp_id = nullptr, *perm = nullptr, *cnt = nullptr;
   mps::u64* part = nullptr;  // tile totals of the scan (int or packed pairs)
   int* nstart = nullptr;     // [tot+2] starts of each bucket's near-wall particles
+  mps::u64* fpart = nullptr; // tile totals of the fluid counts, accumulated by the counting kernels
   // captured step graph
   int graph_steps = 0;
   hipGraph_t graph = nullptr;
@@ -392,7 +393,7 @@ struct Impl : ImplBase {
         (rc = alloc(&perm, scr)) || (rc = alloc(&cnt, (size_t)tot + 1)))
       return rc;
     if ((rc = alloc(&part, ((size_t)tot + 2 + mps::kScanTile - 1) / mps::kScanTile)) ||
-        (rc = alloc(&nstart, (size_t)tot + 2)))
+        (rc = alloc(&nstart, (size_t)tot + 2)) || (rc = alloc(&fpart, fluid_tiles())))
       return rc;
     HIP_TRY(hipMemcpy(dtab, csts.data(), S * sizeof(mps::Cst<real>), hipMemcpyHostToDevice));
 
@@ -487,8 +488,9 @@ struct Impl : ImplBase {
     hipLaunchKernelGGL(mps::k_fill_int, dim3(nblk(tot + 2, 4096)), B, 0, stream, mcs, tot + 2, 0);
     hipLaunchKernelGGL(mps::k_fill_int, dim3(nblk(tot + 2, 4096)), B, 0, stream, mcs2, tot + 2, 0);
     hipLaunchKernelGGL(mps::k_fill_int, dim3(1), B, 0, stream, mcount, 1, 0);
-    // the counting array stays zero between builds (the scan clears it)
-    hipLaunchKernelGGL(mps::k_fill_int, dim3(nblk(tot + 1, 4096)), B, 0, stream, cnt, tot + 1, 0);
+    // the counting array stays zero between builds (the scan clears it, the
+    // fluid scatter the tile totals)
+    clear_counts();
     hipLaunchKernelGGL((mps::k_fill<uint8_t>), dim3(nblk(tot, 4096)), B, 0, stream, mnear, tot,
                        (uint8_t)0);
     hipLaunchKernelGGL((mps::k_fill<uint8_t>), dim3(nblk(tot, 4096)), B, 0, stream, mnear2, tot,
@@ -603,6 +605,17 @@ struct Impl : ImplBase {
     std::swap(fsid[0], fsid[1]);
   }
 
+  // tiles of the fluid bucket scan over tot + 1 counts
+  int fluid_tiles() const { return (tot + 2 + mps::kScanTile - 1) / mps::kScanTile; }
+  mps::FluidCount fcount() {
+    return mps::FluidCount{cnt, fpart, use_cls ? (const uint8_t*)wnear : (const uint8_t*)nullptr, tot};
+  }
+  void clear_counts() {
+    const dim3 B(mps::kBlock);
+    hipLaunchKernelGGL(mps::k_fill_int, dim3(nblk(tot + 1, 4096)), B, 0, stream, cnt, tot + 1, 0);
+    hipLaunchKernelGGL((mps::k_fill<mps::u64>), dim3(1), B, 0, stream, fpart, fluid_tiles(), mps::u64(0));
+  }
+
   // MkBkt (kernel.cu:532-550) as count / scan / scatter / rank / gather, then
   // the scene-uniform workgroup table of the new order.
   int bucket() {
@@ -613,21 +626,25 @@ struct Impl : ImplBase {
       return MPS_OK;
     }
     if (!keys_ready) {
-      if (cnt_dirty)
-        hipLaunchKernelGGL(mps::k_fill_int, dim3(nblk(tot + 1, 4096)), B, 0, stream, cnt, tot + 1, 0);
+      if (cnt_dirty) clear_counts();
       TLAUNCH("bucket_key", (mps::k_bkt_key<real, DIM>), dim3(nblk(nf)), B, 0, stream, fpos[0],
               (const int8_t*)ftyp[0], (const int*)fsid[0], (const mps::Cst<real>*)dtab, nf, tot, key,
-              slot, cnt);
+              slot, fcount());
     }
     keys_ready = cnt_dirty = false;
     // with wall normal classes the scan also gives the near-wall particles'
     // list offsets (nstart) and their count (mcount) for the gather
+    // the counting kernels left the tile totals: only the down-sweep runs
     tick("bucket_scan");
-    if (use_cls) scan(cnt, tot + 1, fcs, wnear, tot, nstart, mcount);
-    else scan(cnt, tot + 1, fcs);
+    {
+      const int nb = fluid_tiles();
+      hipLaunchKernelGGL(mps::k_scan_down<true>, dim3(nb), B, 0, stream, cnt, tot + 1,
+                         nb > 1 ? (const mps::u64*)fpart : nullptr, fcs,
+                         use_cls ? (const uint8_t*)wnear : (const uint8_t*)nullptr, tot, nstart, mcount);
+    }
     tock();
     TLAUNCH("bucket_scatter", mps::k_bkt_scatter, dim3(nblk(nf)), B, 0, stream, key, slot, fid[0], fcs,
-            nf, (const int*)nullptr, 1, tmp, tmp_id, (int*)nullptr);
+            nf, (const int*)nullptr, 1, tmp, tmp_id, (int*)nullptr, fpart, fluid_tiles());
     TLAUNCH("bucket_rank", mps::k_bkt_rank, dim3(nblk(nf)), B, 0, stream, (const int*)tmp,
             (const int*)tmp_id, (const int*)key, (const int*)fcs, nf, tot, perm);
     TLAUNCH("bucket_gather", (mps::k_gather_fluid<real, DIM>), dim3(nblk(nf)), B, 0, stream,
@@ -651,7 +668,7 @@ struct Impl : ImplBase {
   int mirror(bool fresh, bool with_prs = true) {
     const dim3 B(mps::kBlock), SB(block), SG(sb_grid);
     if (cnt_dirty) {  // fluid counts no bucket build has used
-      hipLaunchKernelGGL(mps::k_fill_int, dim3(nblk(tot + 1, 4096)), B, 0, stream, cnt, tot + 1, 0);
+      clear_counts();
       cnt_dirty = keys_ready = false;
     }
     // raw slots: 10 per listed particle (group path) or compacted (general path)
@@ -680,7 +697,7 @@ struct Impl : ImplBase {
     scan(cnt, tot + 1, mcs);
     tock();
     TLAUNCH("mirror_scatter", mps::k_bkt_scatter, dim3(g), B, 0, stream, (const int*)mraw_key, slot,
-            mraw_id, mcs, 0, (const int*)mcount, mult, tmp, tmp_id, (int*)nullptr);
+            mraw_id, mcs, 0, (const int*)mcount, mult, tmp, tmp_id, (int*)nullptr, (mps::u64*)nullptr, 0);
     // live mirrors: mcs[tot]
     TLAUNCH("mirror_gather", (mps::k_rank_gather_mirror<real, DIM>), dim3(g), B, 0, stream, (const int*)tmp,
             (const int*)tmp_id, (const int*)mraw_key, (const int*)mcs, (const int*)(mcs + tot), perm,
@@ -728,7 +745,7 @@ struct Impl : ImplBase {
   mps::Tail tail(bool keys) {
     if (S != 1) return mps::Tail{};
     if (keys) keys_ready = cnt_dirty = true;
-    return mps::Tail{nf, tot, keys ? key : nullptr, slot, cnt};
+    return mps::Tail{nf, tot, keys ? key : nullptr, slot, fcount()};
   }
 
   // d_VscTrm (+ d_UpPcl1 when fuse): fuse writes the new pos/vel and swaps

@@ -335,17 +335,44 @@ __device__ inline int wave_count(int* cnt, int k) {
   return slot;
 }
 
+// The fluid bucket build's counts: per bucket (cnt) and, for the scan, per
+// tile of kScanTile buckets (part: the pair (count, count of the buckets
+// flagged in `flag`) that k_scan_reduce<true> would compute), so the scan
+// needs only its down-sweep.  The tile totals take one atomic per distinct
+// tile of the wavefront.
+struct FluidCount {
+  int* cnt;
+  u64* part;
+  const uint8_t* flag;  // may be null
+  int nflag;
+};
+
+__device__ inline int fluid_count(const FluidCount& fc, int k) {
+  const int slot = wave_count(fc.cnt, k);
+  const int lane = __lane_id(), tile = k / kScanTile;
+  const bool near = k < fc.nflag && fc.flag && fc.flag[k];
+  unsigned long long todo = __ballot(1);
+  while (todo) {
+    const int leader = __ffsll((long long)todo) - 1;
+    const int lt = __shfl(tile, leader);
+    const unsigned long long same = __ballot(tile == lt), nearm = __ballot(tile == lt && near);
+    if (lane == leader) atomicAdd(&fc.part[lt], u64(__popcll(same)) | (u64(__popcll(nearm)) << 32));
+    todo &= ~same;
+  }
+  return slot;
+}
+
 // Work of the lanes past the live range in the fused kernels of a
 // single-scene handle, whose grid then covers the whole fluid set: the
 // deleted tail [live end, n) is carried over to the output buffers, and the
 // kernels that move particles write the next bucket build's key (tot for a
-// deleted particle) and slot as k_bkt_key would, for every entry.  n = 0:
-// none of this (the host copies the tail, the bucket build keys).
+// deleted particle), slot and counts as k_bkt_key would, for every entry.
+// n = 0: none of this (the host copies the tail, the bucket build keys).
 struct Tail {
   int n, tot;
   int* key;  // null: no keys
   int* slot;
-  int* cnt;
+  FluidCount fc;
 };
 
 template <class real, int DIM>
@@ -356,14 +383,13 @@ __device__ inline void next_key(const Tail& tl, int i, bool live, const Vec<real
     k = cell_key(q.ix, q.iy, q.iz, c);
   }
   tl.key[i] = k;
-  tl.slot[i] = wave_count(tl.cnt, k);
+  tl.slot[i] = fluid_count(tl.fc, k);
 }
 
 template <class real, int DIM>
 __global__ void k_bkt_key(const Vec<real, DIM>* __restrict__ pos, const int8_t* __restrict__ typ,
                           const int* __restrict__ sid, const Cst<real>* __restrict__ tab, int n,
-                          int tot, int* __restrict__ key, int* __restrict__ slot,
-                          int* __restrict__ cnt) {
+                          int tot, int* __restrict__ key, int* __restrict__ slot, FluidCount fc) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     int k = tot;
     if (typ[i] != kGST) {
@@ -372,7 +398,7 @@ __global__ void k_bkt_key(const Vec<real, DIM>* __restrict__ pos, const int8_t* 
       k = cell_key(q.ix, q.iy, q.iz, c);
     }
     key[i] = k;
-    slot[i] = wave_count(cnt, k);
+    slot[i] = fluid_count(fc, k);
   }
 }
 
@@ -388,12 +414,15 @@ __global__ void k_key_count(const int* __restrict__ key, const int* __restrict__
 
 // Element count: n_static, or mult * (*n_dyn) when n_dyn is given.  The
 // element's id goes beside it (tmp_id) for the rank kernel.  `clear` (may be
-// null, and not n_dyn) is zeroed.
+// null, and not n_dyn) and clear_part[0..n_part) are zeroed.
 __global__ void k_bkt_scatter(const int* __restrict__ key, const int* __restrict__ slot,
                               const int* __restrict__ id, const int* __restrict__ cs, int n_static,
                               const int* __restrict__ n_dyn, int mult, int* __restrict__ tmp,
-                              int* __restrict__ tmp_id, int* __restrict__ clear) {
+                              int* __restrict__ tmp_id, int* __restrict__ clear, u64* __restrict__ clear_part,
+                              int n_part) {
   if (clear && blockIdx.x == 0 && threadIdx.x == 0) *clear = 0;
+  if (blockIdx.x == 0)  // tile totals the scan has read, cleared for the next count
+    for (int t = threadIdx.x; t < n_part; t += blockDim.x) clear_part[t] = 0;
   const int n = n_dyn ? *n_dyn * mult : n_static;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const int k = key[i];
@@ -1159,7 +1188,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
     vel_out[t] = vel[t];
     if (tl.key) {
       tl.key[t] = tl.tot;
-      tl.slot[t] = wave_count(tl.cnt, tl.tot);
+      tl.slot[t] = fluid_count(tl.fc, tl.tot);
     }
   };
   if (!scene_block(sc, fcs, s, c, first, end)) {  // whole workgroup

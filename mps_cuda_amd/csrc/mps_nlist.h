@@ -468,7 +468,7 @@ __global__ __launch_bounds__(kBlock) void k_grad_list(
     pav[t] += prs[t];
     if (tl.key) {
       tl.key[t] = tl.tot;
-      tl.slot[t] = wave_count(tl.cnt, tl.tot);
+      tl.slot[t] = fluid_count(tl.fc, tl.tot);
     }
   };
   if (!scene_block(sc, fcs, s, c, first, end)) {  // whole workgroup
