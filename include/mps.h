@@ -316,11 +316,18 @@ int mps_probe_arith(int precision, const double* a, const double* b, double* sqr
                     double* div_out, int n);
 
 /* The neighbour kernels' division by a reused divisor (FP32: reciprocal
- * refined to RN(1/b), then one correction per quotient) on n values:
- * rcp_out = the reciprocal it uses, div_out = its a/b.  Used to show that it
- * equals the IEEE quotient bit for bit.  No reference counterpart. */
+ * refined to RN(1/b), then one correction per quotient, for b in
+ * [2^-40, 2^40] and a in [2^-60, 2^60] or 0; IEEE division otherwise) on n
+ * values: rcp_out = the reciprocal it uses, div_out = its a/b.  Used to show
+ * that it equals the IEEE quotient bit for bit.  No reference counterpart. */
 int mps_probe_division(int precision, const double* a, const double* b, double* rcp_out,
                        double* div_out, int n);
+
+/* The neighbour kernels' square root of a squared distance (FP32: the
+ * hardware estimate and its two residual corrections for a in
+ * [2^-40, 2^40], IEEE sqrt otherwise) on n values.  Used to show that it
+ * equals the IEEE square root bit for bit.  No reference counterpart. */
+int mps_probe_sqrt(int precision, const double* a, double* sqrt_out, int n);
 
 #ifdef __cplusplus
 }

@@ -1770,4 +1770,11 @@ int mps_probe_division(int precision, const double* a, const double* b, double* 
                                : probe<float>(a, b, rcp_out, div_out, n, 1);
 }
 
+int mps_probe_sqrt(int precision, const double* a, double* sqrt_out, int n) {
+  if (!a || !sqrt_out || n <= 0) return set_err(MPS_E_ARG, "bad argument");
+  std::vector<double> ones(n, 1.0), q(n);
+  return precision == MPS_FP64 ? probe<double>(a, ones.data(), sqrt_out, q.data(), n, 2)
+                               : probe<float>(a, ones.data(), sqrt_out, q.data(), n, 2);
+}
+
 }  // extern "C"

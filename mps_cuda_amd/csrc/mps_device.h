@@ -171,14 +171,19 @@ struct Divisor;
 template <>
 struct Divisor<float> {
   float b, y;
-  bool ok;
+  // ok: b in [2^-20, 2^20] (up to three chained quotients);  wide: b in
+  // [2^-40, 2^40], enough for one quotient of an a that fits (it lies in
+  // [2^-100, 2^100], its residual is exact).
+  bool ok, wide;
   // The range tests compare bit patterns: for a non-negative float u,
   // (u - lo) <= (hi - lo) as unsigned integers holds exactly for lo <= u <= hi
   // (and fails for NaN and infinity).  0x35800000 = 2^-20, 0x49800000 = 2^20,
-  // 0x21800000 = 2^-60, 0x5d800000 = 2^60.  No branches.
+  // 0x2b800000 = 2^-40, 0x53800000 = 2^40, 0x21800000 = 2^-60,
+  // 0x5d800000 = 2^60.  No branches.
   __device__ explicit Divisor(float b_) : b(b_) {
     const unsigned bits = __float_as_uint(b);
     ok = bits - 0x35800000u <= 0x14000000u;
+    wide = bits - 0x2b800000u <= 0x28000000u;
     const float y0 = __builtin_amdgcn_rcpf(b);
     const float e = __builtin_fmaf(-b, y0, 1.0f);
     const float yfix = __uint_as_float(((253u - ((bits >> 23) & 0xffu)) << 23) | 1u);
@@ -213,13 +218,76 @@ struct Divisor<float> {
 template <>
 struct Divisor<double> {
   double b;
-  bool ok = true;
+  bool ok = true, wide = true;
   __device__ explicit Divisor(double b_) : b(b_) {}
   __device__ static bool fits(double) { return true; }
   __device__ static bool fit3(double, double, double) { return true; }
   __device__ double recip() const { return 1.0 / b; }
   __device__ double div(double a) const { return a / b; }
   __device__ double div_sz(double a) const { return a / b; }
+};
+
+// ---- square root of a squared distance ------------------------------------
+// hipcc's correctly rounded FP32 sqrt takes the hardware estimate s (within
+// one ulp), tests the residuals of its neighbours s - ulp and s + ulp
+// (x - (s -/+ ulp) s, one fma each) and keeps the neighbour the tests
+// select; around that it rescales inputs below 2^-96 and passes zeros,
+// infinities and NaN through.  For x in [2^-40, 2^40] (a squared distance
+// between particles: the distance lies in [2^-20, 2^20], where Divisor's
+// `ok` holds) neither of those applies, and sqrt_mid runs the middle steps
+// alone: bit for bit sqrt(x) there, with seven fewer instructions.  Callers
+// branch on sq_mid and use sqrt outside.  FP64 keeps sqrt.
+__device__ inline bool sq_mid(float x) { return __float_as_uint(x) - 0x2b800000u <= 0x28000000u; }
+__device__ inline bool sq_mid(double) { return true; }
+__device__ inline float sqrt_mid(float x) {
+  const float s = __builtin_amdgcn_sqrtf(x);
+  const float dn = __uint_as_float(__float_as_uint(s) - 1u), up = __uint_as_float(__float_as_uint(s) + 1u);
+  const float t = __builtin_fmaf(-dn, s, x) <= 0.0f ? dn : s;
+  return __builtin_fmaf(-up, s, x) > 0.0f ? up : t;
+}
+__device__ inline double sqrt_mid(double x) { return sqrt(x); }
+
+// The pressure-radius weight WEI_grad(dist, rp) = (rp / dist) - (dist / rp)
+// of dist = sqrt(d2) (const.cuh:46; MkPrs sums it, kernel.cu:1347 and 1377),
+// and the gradient weight that divides it by d2 (PrsGrdTrm, kernel.cu:1541
+// and 1586; the tensor form, kernel.cu:1732 and 1770), bit for bit.  With rp in
+// [2^-20, 2^20] and d2 in [2^-40, 2^40] the quotients take Divisor's path:
+// rp / dist by dist, dist / rp by rp (built once), both operands within
+// [2^-20, 2^20].  The difference w0 of x = RN(rp/dist) and y = RN(dist/rp)
+// is 0 or at least 2^-24 in size (x y lies within 2^-23 of 1, so when the
+// two are within a factor 2 of each other both lie in [1/2, 2] and the
+// subtraction is exact, and otherwise |x - y| > max(x, y) / 2 >= 1/2) and at
+// most 2^40: one quotient by d2, which Divisor's `wide` admits, keeps it
+// correctly rounded; w0 is never -0, so div_sz is exact.
+template <class real>
+struct RpWeight {
+  real rp;
+  Divisor<real> drp;
+  bool fast;
+  __device__ explicit RpWeight(real rp_) : rp(rp_), drp(rp_), fast(drp.ok) {}
+  // ni += (rp / dist) - (dist / rp) when dist < rp
+  __device__ void add_nd(real d2, real& ni) const {
+    if (fast & sq_mid(d2)) {
+      const real dist = sqrt_mid(d2);
+      if (dist < rp) {
+        const Divisor<real> D(dist);
+        ni += D.div_sz(rp) - drp.div_sz(dist);
+      }
+    } else {
+      const real dist = sqrt(d2);
+      if (dist < rp) ni += (rp / dist) - (dist / rp);
+    }
+  }
+  // ((rp / dist) - (dist / rp)) / d2
+  __device__ real grad(real d2) const {
+    if (fast & sq_mid(d2)) {
+      const real dist = sqrt_mid(d2);
+      const Divisor<real> D(dist), D2(d2);
+      return D2.div_sz(D.div_sz(rp) - drp.div_sz(dist));
+    }
+    const real dist = sqrt(d2);
+    return ((rp / dist) - (dist / rp)) / d2;
+  }
 };
 
 // |q|^2 in the reference's operand order (x, then y, then z).

@@ -1209,22 +1209,35 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
 
   // One neighbour at pj with velocity u (0 for a mirror): kernel.cu:1056-1064
   // and 1090-1098, (((2 (u - v)) l0) / dist / dist / dist) per component.
+  // When every active lane's squared distance lies in [2^-40, 2^40] (the
+  // usual case), the wavefront takes sqrt_mid, and each distance lies in
+  // [2^-20, 2^20], where Divisor's `ok` holds; otherwise sqrt and the full
+  // range test.  The choice is uniform over the wavefront, so neither path
+  // diverges on it.
   auto term = [&](const Vec<real, DIM>& pj, const Vec<real, DIM>& u) {
     const real qx = pj.x - px, qy = pj.y - py, qz = vz(pj) - pz;
-    const real dist = sqrt(norm2<real, DIM>(qx, qy, qz));
-    if (dist <= r) {
-      const real nx = real(2) * (u.x - vx) * l0, ny = real(2) * (u.y - vy) * l0;
-      const real nz = DIM == 3 ? real(2) * (vz(u) - vvz) * l0 : real(0);
-      const Divisor<real> D(dist);
-      if (D.ok & D.fit3(nx, ny, nz)) {  // the sums start at +0: div_sz gives them what / gives
-        ax += D.div_sz(D.div_sz(D.div_sz(nx)));
-        ay += D.div_sz(D.div_sz(D.div_sz(ny)));
-        if constexpr (DIM == 3) az += D.div_sz(D.div_sz(D.div_sz(nz)));
-      } else {
-        ax += nx / dist / dist / dist;
-        ay += ny / dist / dist / dist;
-        if constexpr (DIM == 3) az += nz / dist / dist / dist;
+    const real d2 = norm2<real, DIM>(qx, qy, qz);
+    auto body = [&](real dist, bool ok) {
+      if (dist <= r) {
+        const real nx = real(2) * (u.x - vx) * l0, ny = real(2) * (u.y - vy) * l0;
+        const real nz = DIM == 3 ? real(2) * (vz(u) - vvz) * l0 : real(0);
+        const Divisor<real> D(dist);
+        if (ok & D.fit3(nx, ny, nz)) {  // the sums start at +0: div_sz gives them what / gives
+          ax += D.div_sz(D.div_sz(D.div_sz(nx)));
+          ay += D.div_sz(D.div_sz(D.div_sz(ny)));
+          if constexpr (DIM == 3) az += D.div_sz(D.div_sz(D.div_sz(nz)));
+        } else {
+          ax += nx / dist / dist / dist;
+          ay += ny / dist / dist / dist;
+          if constexpr (DIM == 3) az += nz / dist / dist / dist;
+        }
       }
+    };
+    if (__ballot(!sq_mid(d2)) == 0) {
+      body(sqrt_mid(d2), true);
+    } else {
+      const real dist = sqrt(d2);
+      body(dist, Divisor<real>(dist).ok);
     }
   };
   int nch = 0;
@@ -1346,17 +1359,21 @@ __global__ void k_wall_pav(real* __restrict__ pav, const real* __restrict__ prs,
 }
 
 // Arithmetic probe: the device's correctly-rounded sqrt and division, used by
-// the tests to show they agree bit for bit with the host's; with `fast`, the
-// quotient comes from Divisor (as the neighbour kernels use it) and s holds
-// its reciprocal.
+// the tests to show they agree bit for bit with the host's.  fast = 1: the
+// quotient comes from Divisor (as the neighbour kernels use it, over its
+// `wide` range) and s holds its reciprocal; fast = 2: s is the neighbour
+// kernels' square root of a squared distance (sqrt_mid where sq_mid holds).
 template <class real>
 __global__ void k_probe(const real* a, const real* b, real* s, real* q, int n, int fast) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
-    if (fast) {
+    if (fast == 1) {
       const Divisor<real> D(b[i]);
       s[i] = D.recip();
-      q[i] = D.ok && D.fits(a[i]) ? D.div(a[i]) : a[i] / b[i];
+      q[i] = D.wide && D.fits(a[i]) ? D.div(a[i]) : a[i] / b[i];
+    } else if (fast == 2) {
+      s[i] = sq_mid(a[i]) ? sqrt_mid(a[i]) : sqrt(a[i]);
+      q[i] = a[i] / b[i];
     } else {
       s[i] = sqrt(a[i]);
       q[i] = a[i] / b[i];

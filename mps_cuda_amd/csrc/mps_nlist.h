@@ -192,7 +192,8 @@ __global__ __launch_bounds__(kBlock) void k_colprs_list(
   const Vec<real, DIM> p = pos[i], v = vel[i];
   const real px = p.x, py = p.y, pz = vz(p);
   const real vx = v.x, vy = v.y, vvz = vz(v);
-  const real rp = c.rp, lim = c.rp2_hi, rlim2 = c.rlim2;
+  const real lim = c.rp2_hi, rlim2 = c.rlim2;
+  const RpWeight<real> rpw(c.rp);
   real v2x = vx, v2y = vy, v2z = vvz;
   real ni = 0;
   const real mi = c.mass, mj = c.mass;
@@ -217,10 +218,7 @@ __global__ __launch_bounds__(kBlock) void k_colprs_list(
   auto fluid = [&](int j, const Vec<real, DIM>& pj) {
     const real qx = pj.x - px, qy = pj.y - py, qz = vz(pj) - pz;
     const real d2 = norm2<real, DIM>(qx, qy, qz);
-    if (d2 <= lim) {
-      const real dist = sqrt(d2);
-      if (dist < rp) ni += (rp / dist) - (dist / rp);
-    }
+    if (d2 <= lim) rpw.add_nd(d2, ni);
     if (d2 < rlim2) {
       if (!walls) apply(qx, qy, qz, d2, vel[j]);
       fluid_contacts++;
@@ -296,10 +294,7 @@ __global__ __launch_bounds__(kBlock) void k_colprs_list(
                               [&](int, const Vec<real, DIM>& pj) {
       const real qx = pj.x - px, qy = pj.y - py, qz = vz(pj) - pz;
       const real d2 = norm2<real, DIM>(qx, qy, qz);
-      if (d2 <= lim) {
-        const real dist = sqrt(d2);
-        if (dist < rp) ni += (rp / dist) - (dist / rp);
-      }
+      if (d2 <= lim) rpw.add_nd(d2, ni);
       if (d2 < rlim2) apply(qx, qy, qz, d2, mkvec<real, DIM>(0, 0, 0));
     });
   if (!act) {
@@ -409,13 +404,12 @@ __global__ __launch_bounds__(kBlock) void k_prs_list(
   const int i = first + threadIdx.x;
   if (i >= end) return;
   const Vec<real, DIM> p = pos[i];
-  const real px = p.x, py = p.y, pz = vz(p), rp = c.rp, lim = c.rp2_hi;
+  const real px = p.x, py = p.y, pz = vz(p), lim = c.rp2_hi;
+  const RpWeight<real> rpw(c.rp);
   real ni = 0;
   auto add = [&](const Vec<real, DIM>& pj) {
     const real d2 = norm2<real, DIM>(pj.x - px, pj.y - py, vz(pj) - pz);
-    if (!(d2 <= lim)) return;
-    const real dist = sqrt(d2);
-    if (dist < rp) ni += (rp / dist) - (dist / rp);
+    if (d2 <= lim) rpw.add_nd(d2, ni);
   };
   const Cell<real, DIM> q = cell_of<real, DIM>(p, c);
   for_fluid_near<real, DIM>(i, n_alloc, list, count, fcs, q, c, [&](int j) { add(pos[j]); });
@@ -480,7 +474,8 @@ __global__ __launch_bounds__(kBlock) void k_grad_list(
   const int i = act ? first + int(threadIdx.x) : first;
   V* stage = s_stage[threadIdx.x / 64];
   const Vec<real, DIM> p = pos[i];
-  const real px = p.x, py = p.y, pz = vz(p), rp = c.rp, rp2 = c.rp2;
+  const real px = p.x, py = p.y, pz = vz(p), rp2 = c.rp2;
+  const RpWeight<real> rpw(c.rp);
   const real prs_i = prs[i];
   real Prs_min = prs_i;
   real ax = 0, ay = 0, az = 0;
@@ -516,8 +511,7 @@ __global__ __launch_bounds__(kBlock) void k_grad_list(
     const real qx = pj.x - px, qy = pj.y - py, qz = vz(pj) - pz;
     const real dist2 = norm2<real, DIM>(qx, qy, qz);
     if (!(dist2 < rp2)) return;
-    const real dist = sqrt(dist2);
-    const real w = ((rp / dist) - (dist / rp)) / dist2;
+    const real w = rpw.grad(dist2);
     const real d = pr - Prs_min;
     if constexpr (TENSOR) {
       const real wx = w * qx, wy = w * qy, wd = w * d;

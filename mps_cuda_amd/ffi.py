@@ -408,6 +408,18 @@ def probe_division(precision, a, b):
     return y, q
 
 
+def probe_sqrt(precision, a):
+    """The neighbour kernels' square root of a squared distance."""
+    lib = load_library()
+    a = np.ascontiguousarray(a, np.float64)
+    s = np.zeros_like(a)
+    lib.mps_probe_sqrt.restype = C.c_int
+    rc = lib.mps_probe_sqrt(precision, C.c_void_p(a.ctypes.data), C.c_void_p(s.ctypes.data), len(a))
+    if rc:
+        raise MPSError(lib.mps_last_error().decode())
+    return s
+
+
 def probe_arith(precision, a, b):
     lib = load_library()
     a = np.ascontiguousarray(a, np.float64)

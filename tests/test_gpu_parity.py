@@ -54,13 +54,44 @@ def test_reused_divisor_division_is_correctly_rounded():
     want = a / b
     bad = np.nonzero(q.astype(f32).view(np.uint32) != want.view(np.uint32))[0]
     assert len(bad) == 0, f"{len(bad)} quotients differ: a={a[bad[0]]!r} b={b[bad[0]]!r}"
+    # one quotient by a divisor of the wide range [2^-40, 2^40]
+    b = np.exp2(rng.uniform(-40, 40, n)).astype(f32)
+    b[:2] = [f32(2.0 ** -40), f32(2.0 ** 40)]
+    _, q = ffi.probe_division(ffi.FP32, a.astype(np.float64), b.astype(np.float64))
+    want = a / b
+    bad = np.nonzero(q.astype(f32).view(np.uint32) != want.view(np.uint32))[0]
+    assert len(bad) == 0, f"{len(bad)} wide-range quotients differ: a={a[bad[0]]!r} b={b[bad[0]]!r}"
     # outside the fast range the division falls back to IEEE
-    a2 = np.array([1e-30, 1.0, 1.0, 3.0, 0.0, 1.0, -2.5e30], f32)
-    b2 = np.array([3.0, 0.0, 1e-10, 1e10, 2.0, 2.0 ** 21, 0.7], f32)
+    a2 = np.array([1e-30, 1.0, 1.0, 3.0, 0.0, 1.0, -2.5e30, 1.0], f32)
+    b2 = np.array([3.0, 0.0, 1e-13, 1e13, 2.0, 2.0 ** 41, 0.7, 2.0 ** -41], f32)
     with np.errstate(divide="ignore", invalid="ignore"):
         want2 = a2 / b2
     _, q2 = ffi.probe_division(ffi.FP32, a2.astype(np.float64), b2.astype(np.float64))
     assert np.array_equal(q2.astype(f32).view(np.uint32), want2.view(np.uint32))
+
+
+def test_distance_sqrt_is_correctly_rounded():
+    """The neighbour kernels' FP32 square root of a squared distance (the
+    hardware estimate and its two residual corrections, for inputs in
+    [2^-40, 2^40]) equals IEEE sqrt bit for bit: every float of 22 binades
+    spread over that range (its first and last included), random inputs
+    across it, and the IEEE fallback outside it (tiny, subnormal, zero,
+    huge, infinite inputs)."""
+    f32 = np.float32
+    sig = np.arange(1 << 23, dtype=np.uint32)
+    for e in list(range(-40, 40, 4)) + [-37, 39]:
+        x = ((sig | np.uint32((127 + e) << 23)).view(f32))
+        got = ffi.probe_sqrt(ffi.FP32, x.astype(np.float64)).astype(f32)
+        bad = np.nonzero(got.view(np.uint32) != np.sqrt(x).view(np.uint32))[0]
+        assert len(bad) == 0, f"{len(bad)} square roots differ in binade 2^{e}: first x={x[bad[0]]!r}"
+    rng = np.random.default_rng(2)
+    x = np.exp2(rng.uniform(-40, 40, 4_000_000)).astype(f32)
+    got = ffi.probe_sqrt(ffi.FP32, x.astype(np.float64)).astype(f32)
+    assert np.array_equal(got.view(np.uint32), np.sqrt(x).view(np.uint32))
+    edge = np.array([0.0, 1e-45, 1e-40, 2.0 ** -41, 2.0 ** -100, 2.0 ** 41, 3e38, np.inf,
+                     2.0 ** -40, 2.0 ** 40], f32)
+    got = ffi.probe_sqrt(ffi.FP32, edge.astype(np.float64)).astype(f32)
+    assert np.array_equal(got.view(np.uint32), np.sqrt(edge).view(np.uint32))
 
 
 CASES = [
