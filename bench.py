@@ -63,7 +63,7 @@ def algorithmic_bytes(kernel: str, nf: int, nw: int, m: int) -> float:
 KERNEL_SYMBOL = {  # timing name -> kernel symbol in the rocprofv3 summaries
     "vsc_up1": "mps::k_vsc<float, 3, true>",
     "col": "mps::k_col_list<float, 3, false>",
-    "col_prs": "mps::k_colprs_list<float, 3>",
+    "col_prs": "mps::k_colprs_list<float, 3, true>",  # the step's form (it also builds the list)
     "prs": "mps::k_prs_list<float, 3>",
     "grad_up2": "mps::k_grad_list<float, 3, true, false>",
     "mirror_gen": "mps::k_mirror_group<float, 3>",
