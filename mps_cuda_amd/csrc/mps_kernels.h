@@ -154,21 +154,29 @@ __device__ inline void scan_row_staged(V* stage, int b, int e, bool act, const V
 constexpr int kScanIPT = 8;
 constexpr int kScanTile = kBlock * kScanIPT;
 
+// Exclusive scan over a kBlock-thread workgroup (every thread calls it);
+// total = the sum of all v.  Each wavefront scans its 64 values by lane
+// shuffles; the wavefront totals meet in sh[0 .. kBlock/64) (one barrier),
+// and a second barrier frees sh for the next call.
 template <class T>
 __device__ inline T block_exclusive_scan(T v, T* sh, T& total) {
-  const int t = threadIdx.x;
-  sh[t] = v;
-  __syncthreads();
-  for (int off = 1; off < kBlock; off <<= 1) {
-    T x = t >= off ? sh[t - off] : T(0);
-    __syncthreads();
-    sh[t] += x;
-    __syncthreads();
+  const int lane = __lane_id(), w = threadIdx.x / 64;
+  T x = v;
+  for (int off = 1; off < 64; off <<= 1) {
+    const T y = __shfl_up(x, off);
+    if (lane >= off) x += y;
   }
-  total = sh[kBlock - 1];
-  T excl = sh[t] - v;
+  if (lane == 63) sh[w] = x;
   __syncthreads();
-  return excl;
+  T before = 0;
+  total = 0;
+  for (int k = 0; k < kBlock / 64; k++) {
+    const T s = sh[k];
+    if (k < w) before += s;
+    total += s;
+  }
+  __syncthreads();
+  return before + x - v;
 }
 
 // Scanned value of count c at index k: c, or with `flag` the pair
