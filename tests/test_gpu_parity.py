@@ -94,6 +94,31 @@ def test_distance_sqrt_is_correctly_rounded():
     assert np.array_equal(got.view(np.uint32), np.sqrt(edge).view(np.uint32))
 
 
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+@pytest.mark.parametrize("grad", [ffi.GRAD_STANDARD, ffi.GRAD_TENSOR])
+def test_near_coincident_pair_takes_the_ieee_paths(oracle_mod, prec, grad):
+    """A lattice block with one particle 1e-7 m from the centre one: their
+    squared distance (1e-14) lies below 2^-40, so the FP32 neighbour kernels
+    take their IEEE square root and division for that pair (VscTrm's
+    wavefront falls back to sqrt, MkPrs's and PrsGrdTrm's weights to plain
+    division).  Bit-exact phase by phase and over fused steps."""
+    from tests.test_oracle import lattice, params_of, scene_of
+    pts = lattice(3, 3)  # 7^3 particles at l0 spacing; index 171 is the centre
+    pts = np.concatenate([pts, pts[171:172] + [1e-7, 0.0, 0.0]])
+    scene = scene_of(3, pts)
+    params = params_of(3, prec, grad_model=grad)
+    g, o = _pair(oracle_mod, scene, params)
+    for k, ph in enumerate(ffi.STEP_PHASES):
+        g.phase(ph)
+        o.phase(ph)
+        assert_states_equal(g.state(), o.state(), f"near pair {prec} phase#{k} ({ph})")
+    for chunk in (1, 3):
+        g.step(chunk)
+        o.step(chunk)
+        assert_states_equal(g.state(), o.state(), f"near pair {prec} after +{chunk} steps",
+                            fields=("typ", "pos", "vel", "prs", "pav", "nd"))
+
+
 CASES = [
     ("small2d-fp64", lambda: configs.small2d("fp64")),
     ("small2d-fp32", lambda: configs.small2d("fp32")),
