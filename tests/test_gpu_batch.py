@@ -66,6 +66,28 @@ def test_batch_steps_equal_single_scenes(name, make):
         _assert_bucket_blocks(batch, singles, f"{name} after {done} steps")
 
 
+def test_batch_of_more_than_256_scenes():
+    """300 scenes: the scene-uniform workgroup table is built 256 scenes at a
+    time with a carry (k_scene_blocks), and the scene lookup spans both
+    passes.  Scenes on both sides of that boundary step exactly as alone."""
+    scenes, params = configs.batch_small(300, "fp32", seed=61)
+    print(f"[case] batch of {len(scenes)} scenes, {sum(s.n for s in scenes)} particles", flush=True)
+    batch = ffi.Handle(params, scenes)
+    pick = (0, 1, 255, 256, 257, 299)
+    singles = {k: ffi.Handle(params[k], scenes[k]) for k in pick}
+    done = 0
+    for chunk in (1, 6):
+        batch.step(chunk)
+        for h in singles.values():
+            h.step(chunk)
+        done += chunk
+        st = batch.state()
+        for k, h in singles.items():
+            a, b = batch.offsets[k], batch.offsets[k + 1]
+            assert_states_equal(_slice(st, a, b), h.state(), f"300-scene batch after {done} steps, scene {k}",
+                                STEP_FIELDS)
+
+
 def test_batch_phases_equal_single_scenes():
     scenes, params = configs.batch_small(4, "fp64", seed=21)
     print(f"[case] batch phases sizes={[s.n for s in scenes]}", flush=True)
