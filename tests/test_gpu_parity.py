@@ -119,6 +119,31 @@ def test_near_coincident_pair_takes_the_ieee_paths(oracle_mod, prec, grad):
                             fields=("typ", "pos", "vel", "prs", "pav", "nd"))
 
 
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_micro_scale_takes_the_ieee_paths(oracle_mod, prec):
+    """l0 = 1e-7 m: rp (2.1e-7) lies below 2^-20 and every squared distance
+    below 2^-40, so the FP32 kernels use IEEE sqrt and division for every
+    neighbour (the reused-divisor and sqrt_mid paths are off).  Bit-exact
+    phase by phase and over fused steps."""
+    from tests.test_oracle import lattice, scene_of
+    l0 = 1e-7
+    pts = lattice(3, 3, spacing=l0)
+    pts[:, 2] += 0.3 * l0 * np.sin(np.arange(len(pts)))
+    scene = scene_of(3, pts, extent=2e-6)
+    params = ffi.default_params(dim=3, precision=prec, l0=l0, range_min=(-2e-6,) * 3, range_max=(2e-6,) * 3,
+                                **dict(configs.COMMON, dt=1e-10))
+    g, o = _pair(oracle_mod, scene, params)
+    for k, ph in enumerate(ffi.STEP_PHASES):
+        g.phase(ph)
+        o.phase(ph)
+        assert_states_equal(g.state(), o.state(), f"micro {prec} phase#{k} ({ph})")
+    for chunk in (1, 5):
+        g.step(chunk)
+        o.step(chunk)
+        assert_states_equal(g.state(), o.state(), f"micro {prec} after +{chunk} steps",
+                            fields=("typ", "pos", "vel", "prs", "pav", "nd"))
+
+
 CASES = [
     ("small2d-fp64", lambda: configs.small2d("fp64")),
     ("small2d-fp32", lambda: configs.small2d("fp32")),
