@@ -733,19 +733,21 @@ struct Impl : ImplBase {
   }
 
   // ---- phase launchers ------------------------------------------------------
-  void copy_tail(V* a_in, V* a_out, V* b_in, V* b_out, real* s_in, real* s_out, real* pav) {
+  // keys: the batch's movers keyed their live particles; key the tail too
+  void copy_tail(V* a_in, V* a_out, V* b_in, V* b_out, real* s_in, real* s_out, real* pav,
+                 bool keys = false) {
+    const mps::Tail tl{0, tot, keys ? key : nullptr, slot, fcount()};
     hipLaunchKernelGGL((mps::k_copy_tail<real, DIM>), dim3(nblk(nf, 1024)), dim3(mps::kBlock), 0, stream,
                        (const int*)fcs, tot, nf, (const V*)a_in, a_out, (const V*)b_in, b_out,
-                       (const real*)s_in, s_out, pav);
+                       (const real*)s_in, s_out, pav, tl);
   }
 
   // Lanes past the live range carry the deleted tail over (single-scene
-  // handles); `keys`: the kernel moves particles and writes the next bucket
-  // build's keys and counts.
+  // handles; a batch's tail goes through copy_tail); `keys`: the kernel
+  // moves particles and writes the next bucket build's keys and counts.
   mps::Tail tail(bool keys) {
-    if (S != 1) return mps::Tail{};
     if (keys) keys_ready = cnt_dirty = true;
-    return mps::Tail{nf, tot, keys ? key : nullptr, slot, fcount()};
+    return mps::Tail{S == 1 ? nf : 0, tot, keys ? key : nullptr, slot, fcount()};
   }
 
   // d_VscTrm (+ d_UpPcl1 when fuse): fuse writes the new pos/vel and swaps
@@ -756,7 +758,7 @@ struct Impl : ImplBase {
       TLAUNCH("vsc_up1", (mps::k_vsc<real, DIM, true>), SG, SB, 0, stream, fpos[0], fvel[0],
               (const int*)fcs, sc(), mpos, mcs, (const uint8_t*)mnear, chunks, nf, (V*)nullptr, fpos[1],
               fvel[1], ftyp[0], fprs[0], tail(true));
-      if (S > 1) copy_tail(fpos[0], fpos[1], fvel[0], fvel[1], nullptr, nullptr, nullptr);
+      if (S > 1) copy_tail(fpos[0], fpos[1], fvel[0], fvel[1], nullptr, nullptr, nullptr, true);
       std::swap(fpos[0], fpos[1]);
       std::swap(fvel[0], fvel[1]);
     } else {
@@ -815,7 +817,7 @@ struct Impl : ImplBase {
         TLAUNCH("grad_up2", (mps::k_grad_list<real, DIM, true, true>), SG, SB, 0, stream, fpos[0], fvel[0],
                 fprs[0], (const int*)fcs, nf, nlist, ncount, mpos, mprs, mcs, (const uint8_t*)mnear, sc(),
                 (V*)nullptr, fpos[1], fvel[1], fprs[1], fpav[0], ftyp[0], tail(true));
-        if (S > 1) copy_tail(fpos[0], fpos[1], fvel[0], fvel[1], fprs[0], fprs[1], fpav[0]);
+        if (S > 1) copy_tail(fpos[0], fpos[1], fvel[0], fvel[1], fprs[0], fprs[1], fpav[0], true);
         std::swap(fpos[0], fpos[1]);
         std::swap(fvel[0], fvel[1]);
         std::swap(fprs[0], fprs[1]);
@@ -831,7 +833,7 @@ struct Impl : ImplBase {
       TLAUNCH("grad_up2", (mps::k_grad_list<real, DIM, true>), SG, SB, 0, stream, fpos[0], fvel[0],
               fprs[0], (const int*)fcs, nf, nlist, ncount, mpos, mprs, mcs, (const uint8_t*)mnear, sc(),
               (V*)nullptr, fpos[1], fvel[1], fprs[1], fpav[0], ftyp[0], tail(true));
-      if (S > 1) copy_tail(fpos[0], fpos[1], fvel[0], fvel[1], fprs[0], fprs[1], fpav[0]);
+      if (S > 1) copy_tail(fpos[0], fpos[1], fvel[0], fvel[1], fprs[0], fprs[1], fpav[0], true);
       std::swap(fpos[0], fpos[1]);
       std::swap(fvel[0], fvel[1]);
       std::swap(fprs[0], fprs[1]);
@@ -929,7 +931,7 @@ struct Impl : ImplBase {
       int rc;
       if (!gexec && (rc = capture())) return rc;
       for (; nsteps >= graph_steps; nsteps -= graph_steps) HIP_TRY(hipGraphLaunch(gexec, stream));
-      keys_ready = cnt_dirty = S == 1;  // as the graph's last step leaves them
+      keys_ready = cnt_dirty = nf > 0;  // as the graph's last step leaves them
     }
     step_eager(nsteps);
     return check_launch();

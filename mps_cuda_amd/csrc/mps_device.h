@@ -133,7 +133,10 @@ __device__ inline bool scene_block(const Scenes<real>& sc, const int* __restrict
     end = cs[c.cbase + c.ncell];
     return first < end;
   }
-  if (b >= sc.blk_off[sc.S]) return false;
+  if (b >= sc.blk_off[sc.S]) {  // an empty range: the callers' tail lanes then do nothing
+    first = end = 0;
+    return false;
+  }
   int lo = 0, hi = sc.S;
   while (hi - lo > 1) {
     const int mid = (lo + hi) >> 1;

@@ -370,12 +370,12 @@ __device__ inline int fluid_count(const FluidCount& fc, int k) {
   return slot;
 }
 
-// Work of the lanes past the live range in the fused kernels of a
-// single-scene handle, whose grid then covers the whole fluid set: the
-// deleted tail [live end, n) is carried over to the output buffers, and the
-// kernels that move particles write the next bucket build's key (tot for a
-// deleted particle), slot and counts as k_bkt_key would, for every entry.
-// n = 0: none of this (the host copies the tail, the bucket build keys).
+// The kernels that move particles write the next bucket build's key (tot
+// for a deleted particle), slot and counts as k_bkt_key would, when key is
+// set.  In a single-scene handle (n = the fluid set's size) their grid
+// covers the whole fluid set, and the lanes past the live range carry the
+// deleted tail [live end, n) over to the output buffers and key it.  With
+// n = 0 (a batch) k_copy_tail does that.
 struct Tail {
   int n, tot;
   int* key;  // null: no keys
@@ -565,19 +565,24 @@ __global__ void k_mirror_prs(const int* __restrict__ n_dyn, const int* __restric
 }
 
 // Copies the deleted tail [cs[tot], n) of double-buffered fields (fused
-// kernels only visit live particles); with `s_in` it also adds prs to pav.
+// kernels only visit live particles); with `s_in` it also adds prs to pav;
+// with tl.key it keys the tail for the next bucket build (key tot).
 template <class real, int DIM>
 __global__ void k_copy_tail(const int* __restrict__ cs, int tot, int n,
                             const Vec<real, DIM>* __restrict__ a_in, Vec<real, DIM>* __restrict__ a_out,
                             const Vec<real, DIM>* __restrict__ b_in, Vec<real, DIM>* __restrict__ b_out,
                             const real* __restrict__ s_in, real* __restrict__ s_out,
-                            real* __restrict__ pav) {
+                            real* __restrict__ pav, Tail tl) {
   for (int i = cs[tot] + blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     if (a_out) a_out[i] = a_in[i];
     if (b_out) b_out[i] = b_in[i];
     if (s_out) {
       s_out[i] = s_in[i];
       pav[i] += s_in[i];
+    }
+    if (tl.key) {
+      tl.key[i] = tot;
+      tl.slot[i] = fluid_count(tl.fc, tot);
     }
   }
 }
